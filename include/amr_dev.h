@@ -1,0 +1,166 @@
+/*
+ * amr_dev.h -- C ABI of the MI355X (gfx950) miniAMR hot path.
+ *
+ * The reference program has no plugin or FFI layer: its per-timestep data
+ * path is five void functions over global state, declared in ref/proto.h.
+ * This header is the drop-in seam that replaces them.  The caller (the C
+ * driver in miniamr_amd/host, or ctypes from Python) owns the block/object
+ * model, the refinement decisions and the slot allocator; the library owns
+ * the cell values, which stay resident in device memory for the lifetime of
+ * the context.
+ *
+ *   reference call site                         replaced by
+ *   -----------------------------------------   --------------------------
+ *   allocate()          ref/main.c:418-605      amr_dev_create
+ *   deallocate()        ref/main.c:609-624      amr_dev_destroy
+ *   value fill in init  ref/init.c:484-495      amr_dev_upload_block
+ *   comm(start,n,stage) ref/comm.c:42-242       amr_dev_set_mesh + amr_dev_comm
+ *     on_proc_comm      ref/comm.c:1465-1580      (face jobs, kind SAME)
+ *     on_proc_comm_diff ref/comm.c:1585-1908      (kinds PROLONG, RESTRICT)
+ *     apply_bc          ref/comm.c:1911-1965      (kind BC)
+ *   stencil_driver(var,stage) ref/stencil.c:43-145  amr_dev_stencil
+ *   check_sum(var)      ref/check_sum.c:36-65    amr_dev_check_sum
+ *   split_blocks() data loop ref/block.c:161-173   amr_dev_split
+ *   consolidate_blocks() data loop ref/block.c:411-431  amr_dev_consolidate
+ *
+ * Conventions
+ *   - Cell arrays use the reference's index order array[var][i][j][k]
+ *     (ref/block.h:52): i = x, j = y, k = z (fastest), one ghost layer,
+ *     interior indices 1..n.
+ *   - Faces are numbered 0..5 = W, E, S, N, D, U (ref/block.h:49).
+ *   - Every function returns AMR_OK (0) or a negative AMR_ERR_* code; the
+ *     text of the last error is available from amr_dev_last_error.
+ *   - One HIP stream per context; calls on one context must not overlap
+ *     (the reference's functions are not reentrant either).  Functions that
+ *     return values to the host (check_sum, download) synchronise the
+ *     stream; the others only enqueue work.
+ */
+#ifndef AMR_DEV_H
+#define AMR_DEV_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct amr_dev amr_dev;
+
+enum {
+   AMR_OK = 0,
+   AMR_ERR_ARG = -1,     /* bad argument (size, slot, var out of range) */
+   AMR_ERR_HIP = -2,     /* HIP runtime error */
+   AMR_ERR_NOMEM = -3,   /* device allocation failed */
+   AMR_ERR_STATE = -4,   /* call out of order (e.g. comm before set_mesh) */
+   AMR_ERR_NODEV = -5    /* no usable gfx950 device */
+};
+
+/* Storage precision of the cell values. */
+enum {
+   AMR_PREC_F64 = 0,     /* reference arithmetic, bit-exact */
+   AMR_PREC_F32 = 1,     /* FP32 storage, FP32 sums, FP64 checksums */
+   AMR_PREC_BF16 = 2     /* BF16 storage, FP32 sums, FP64 checksums */
+};
+
+/* Face-job kinds of one exchange direction (code 0 semantics). */
+enum {
+   AMR_FACE_SAME = 0,     /* dst ghost plane <- src boundary plane
+                             (ref/comm.c:1473-1534) */
+   AMR_FACE_PROLONG = 1,  /* dst is fine: ghost <- coarse cell / 4
+                             (ref/comm.c:1618-1625 and the y/z twins) */
+   AMR_FACE_RESTRICT = 2, /* dst is coarse: ghost quarter <- sum of 4 fine
+                             cells (ref/comm.c:1626-1630 and twins) */
+   AMR_FACE_BC = 3        /* reflective boundary (ref/comm.c:1911-1965) */
+};
+
+typedef struct {
+   int nx, ny, nz;     /* interior block size, each even and > 0 */
+   int num_vars;       /* > 0 */
+   int max_blocks;     /* number of block slots (ref: --max_blocks) */
+   int stencil;        /* 7 or 27 */
+   int precision;      /* AMR_PREC_* */
+   int device;         /* HIP device ordinal */
+} amr_dev_config;
+
+/* One ghost-plane fill.  `face` is the face of `dst` whose ghost plane is
+ * written.  SAME/PROLONG/RESTRICT read `src`; BC reads `dst` itself.
+ * (iq, jq) is the quarter of the coarse block's face that the fine block
+ * covers, as in ref/comm.c:1585 (iq selects the faster in-plane axis). */
+typedef struct {
+   int32_t kind, face, dst, src, iq, jq;
+} amr_face_job;
+
+/* child interior <- parent octant / 8  (ref/block.c:161-173).
+ * Octant o: x half o%2, y half (o/2)%2, z half o/4. */
+typedef struct {
+   int32_t parent, child, octant, pad;
+} amr_split_job;
+
+/* dst octant <- sum of each 2x2x2 group of child cells
+ * (ref/block.c:411-431), same octant numbering. */
+typedef struct {
+   int32_t dst, child, octant, pad;
+} amr_merge_job;
+
+int amr_dev_create(const amr_dev_config *cfg, amr_dev **out);
+int amr_dev_destroy(amr_dev *dev);
+const char *amr_dev_last_error(const amr_dev *dev);
+
+/* Copy one slot's cells, [num_vars][nx+2][ny+2][nz+2] doubles, to/from the
+ * device (converted to/from the storage precision). */
+int amr_dev_upload_block(amr_dev *dev, int slot, const double *cells);
+int amr_dev_download_block(amr_dev *dev, int slot, double *cells);
+
+/* Install the active-block list and the exchange plan after each change of
+ * the mesh.  sorted_slots[n_active] lists the active slots in the
+ * reference's sorted-list order (ref/block.h:72-77), which fixes the
+ * checksum summation order.  jobs[] holds dir_count[0] x-direction jobs,
+ * then dir_count[1] y-direction jobs, then dir_count[2] z-direction jobs. */
+int amr_dev_set_mesh(amr_dev *dev, const int32_t *sorted_slots, int n_active,
+                     const amr_face_job *jobs, const int32_t dir_count[3]);
+
+/* Ghost exchange for variables [start, start+num_comm): three ordered
+ * phases x, y, z, or permutations[stage % 6] when permute != 0
+ * (ref/comm.c:45-55). */
+int amr_dev_comm(amr_dev *dev, int start, int num_comm, int stage, int permute);
+
+/* 7- or 27-point averaging stencil of one variable over all active blocks. */
+int amr_dev_stencil(amr_dev *dev, int var);
+
+/* Sum of one variable's interior cells over all active blocks (FP64). */
+int amr_dev_check_sum(amr_dev *dev, int var, double *sum);
+
+/* Data half of split_blocks / consolidate_blocks.  Jobs are applied as if
+ * one after another in array order (a slot freed by an earlier job may be
+ * the target of a later one). */
+int amr_dev_split(amr_dev *dev, const amr_split_job *jobs, int n);
+int amr_dev_consolidate(amr_dev *dev, const amr_merge_job *jobs, int n);
+
+/* Device-side checksum table: n_rows rows of num_vars doubles that
+ * amr_dev_stage fills, so a run can enqueue many stages and read the sums
+ * back once.  Rows are indexed by the caller (e.g. checksum event number
+ * within a refinement interval). */
+int amr_dev_reserve_sums(amr_dev *dev, int n_rows);
+int amr_dev_read_sums(amr_dev *dev, int row0, int n_rows, double *out);
+
+/* One stage for variables [start, start+num): the three exchange phases
+ * (as amr_dev_comm) then the stencil of each variable; when sum_row >= 0
+ * the checksum of each variable is stored in row sum_row of the checksum
+ * table (columns start..start+num-1).  Enqueue only. */
+int amr_dev_stage(amr_dev *dev, int start, int num, int stage, int permute,
+                  int sum_row);
+
+/* Wait for all enqueued work. */
+int amr_dev_sync(amr_dev *dev);
+
+/* Device time of the kernels enqueued since the last call, in
+ * milliseconds, split as [0] exchange, [1] stencil, [2] checksum,
+ * [3] split/consolidate.  Requires amr_dev_set_timing(dev, 1). */
+int amr_dev_set_timing(amr_dev *dev, int on);
+int amr_dev_read_timing(amr_dev *dev, double ms[4]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AMR_DEV_H */

@@ -1,0 +1,327 @@
+/*
+ * driver.c -- the timestep loop of the drop-in (ref/driver.c:37-214).
+ *
+ * The reference runs, per stage, "exchange a group of variables, then
+ * stencil and checksum each variable" and tests conservation right after
+ * each checksum.  Variables are independent (the stencil and every
+ * exchange rule act on one variable at a time), so the order in which
+ * (stage, variable) pairs are computed does not change any value.  This
+ * driver therefore enqueues a whole timestep on the device without
+ * waiting, in one of two orders:
+ *   stage-major (default): stage by stage, all variables per launch, as
+ *                          the reference does;
+ *   variable-major (MINIAMR_SCHEDULE=var): a group of variables through all
+ *                          stages of the timestep, then the next group, so
+ *                          a group's cells stay in the device's last-level
+ *                          cache across stages.
+ * The checksums of the timestep land in a device table and are then
+ * checked on the host in the reference's order, printing the same lines.
+ * The only observable difference is after a failed conservation check:
+ * the reference stops mid-stage, while here the rest of that timestep's
+ * cell updates have already run (the printed output and the Summary line
+ * are the same).
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "amr.h"
+
+double amr_checksum(amr_run *R, int var)
+{
+   double s = 0.0;
+   amr_dev_check(R, amr_dev_check_sum(R->dev, var, &s), "check_sum");
+   R->total_red++;
+   return s;
+}
+
+static FILE *g_sum_log;
+
+static void log_sum(double s)
+{
+   if (g_sum_log) {
+      fprintf(g_sum_log, "%.17g\n", s);
+      fflush(g_sum_log);
+   }
+}
+
+/* ref/driver.c:171-214 */
+double amr_calc_time_step(amr_run *R)
+{
+   const amr_params *P = &R->P;
+   double delta, tmp, inv_cell_size[3];
+   if (P->use_tsteps)
+      return P->end_time / (double) P->num_tsteps;
+   inv_cell_size[0] = (double) (R->mesh_size[0] * P->nx);
+   inv_cell_size[1] = (double) (R->mesh_size[1] * P->ny);
+   inv_cell_size[2] = (double) (R->mesh_size[2] * P->nz);
+   delta = 0.0;
+   for (int o = 0; o < P->num_objects; o++) {
+      const amr_object *op = &P->objects[o];
+      int done = 0;
+      if (op->size[0] < 0.0 || op->size[1] < 0.0 || op->size[2] < 0.0)
+         break;
+      for (int d = 0; d < 3; d++)
+         if (op->cen[d] < 0.0) {
+            if (op->cen[d] + op->size[d] < 0.0)
+               done = 1;
+         } else if (op->cen[d] > 1.0) {
+            if (op->cen[d] - op->size[d] > 1.0)
+               done = 1;
+         }
+      if (done)
+         break;
+      for (int d = 0; d < 3; d++) {
+         tmp = (fabs(op->move[d]) + fabs(op->inc[d])) * inv_cell_size[d];
+         if (tmp > delta)
+            delta = tmp;
+      }
+   }
+   return delta > 0.0 ? 1.0 / delta : 1.0;
+}
+
+/* Enqueue one timestep; checksum event e of the timestep goes to row e. */
+static void enqueue_timestep(amr_run *R, int comm_stage0)
+{
+   const amr_params *P = &R->P;
+   int V = P->num_vars;
+   if (P->schedule == 0) {
+      for (int stage = 0; stage < P->stages_per_ts; stage++) {
+         int row = (P->checksum_freq && !(stage % P->checksum_freq))
+                      ? stage / P->checksum_freq : -1;
+         for (int start = 0; start < V; start += P->comm_vars) {
+            int number = (start + P->comm_vars > V) ? V - start : P->comm_vars;
+            amr_dev_check(R, amr_dev_stage(R->dev, start, number, comm_stage0 + stage,
+                                           P->permute, row), "stage");
+         }
+      }
+   } else {
+      int group = P->comm_vars;
+      const char *g = getenv("MINIAMR_VAR_GROUP");
+      if (g && atoi(g) > 0)
+         group = atoi(g);
+      for (int start = 0; start < V; start += group) {
+         int number = (start + group > V) ? V - start : group;
+         for (int stage = 0; stage < P->stages_per_ts; stage++) {
+            int row = (P->checksum_freq && !(stage % P->checksum_freq))
+                         ? stage / P->checksum_freq : -1;
+            amr_dev_check(R, amr_dev_stage(R->dev, start, number, comm_stage0 + stage,
+                                           P->permute, row), "stage");
+         }
+      }
+   }
+}
+
+/* Check this timestep's checksums in the reference's order
+ * (ref/driver.c:88-103).  Returns 1 if the run must stop. */
+static int check_timestep(amr_run *R, int ts, const double *sums, int64_t *blocks_done)
+{
+   const amr_params *P = &R->P;
+   int V = P->num_vars;
+   for (int stage = 0; stage < P->stages_per_ts; stage++) {
+      *blocks_done += R->global_active;
+      if (!(P->checksum_freq && !(stage % P->checksum_freq)))
+         continue;
+      const double *row = sums + (size_t) (stage / P->checksum_freq) * V;
+      for (int var = 0; var < V; var++) {
+         double sum = row[var];
+         R->total_red++;
+         log_sum(sum);
+         if (P->report_diffusion)
+            printf("%d var %d sum %lf old %lf diff %lf %lf tol %lf\n", ts, var, sum,
+                   R->grid_sum[var], (sum - R->grid_sum[var]),
+                   (fabs(sum - R->grid_sum[var]) / R->grid_sum[var]), R->tol);
+         if (P->stencil || var == 0)
+            if (fabs(sum - R->grid_sum[var]) / R->grid_sum[var] > R->tol) {
+               printf("Time step %d sum %lf (old %lf) variable %d difference too large\n",
+                      ts, sum, R->grid_sum[var], var);
+               return 1;
+            }
+         R->grid_sum[var] = sum;
+      }
+   }
+   return 0;
+}
+
+/* ref/driver.c:37-165 */
+int amr_driver(amr_run *R)
+{
+   amr_params *P = &R->P;
+   const char *sp = getenv("AMR_DUMP_SUMS");
+   g_sum_log = sp ? fopen(sp, "w") : NULL;
+
+   double t2 = amr_now();
+   amr_init(R);
+   for (int v = 0; v < P->num_vars; v++)
+      log_sum(R->grid_sum[v]);
+   double t1 = amr_now();
+   R->t_init = t1 - t2;
+
+   if (P->num_refine || P->uniform_refine)
+      amr_refine(R, 0);
+   amr_dev_check(R, amr_dev_sync(R->dev), "sync");
+   t2 = amr_now();
+   R->t_refine_all += t2 - t1;
+   R->nb_min = R->nb_max = R->global_active;
+
+   int n_rows = P->checksum_freq ? (P->stages_per_ts + P->checksum_freq - 1) / P->checksum_freq : 0;
+   double *sums = (double *) amr_xmalloc(sizeof(double) * (size_t) (n_rows ? n_rows : 1) *
+                                         P->num_vars);
+   amr_dev_check(R, amr_dev_reserve_sums(R->dev, n_rows ? n_rows : 1), "reserve_sums");
+   amr_dev_set_timing(R->dev, 1);
+   amr_dev_read_timing(R->dev, (double[4]){0});
+
+   double delta = 1.0, sim_time = 0.0, dev_ms[4] = { 0, 0, 0, 0 };
+   if (P->use_time)
+      delta = amr_calc_time_step(R);
+   int comm_stage = 0, done = 0, ts;
+   int failed = 0;
+   for (ts = 1; !done; ts++) {
+      if (P->report_perf & 8)
+         printf("Timestep %d time %lf delta %lf\n", ts, sim_time, delta);
+      enqueue_timestep(R, comm_stage);
+      comm_stage += P->stages_per_ts;
+      if (n_rows)
+         amr_dev_check(R, amr_dev_read_sums(R->dev, 0, n_rows, sums), "read_sums");
+      else
+         amr_dev_check(R, amr_dev_sync(R->dev), "sync");
+      double t3 = amr_now();
+      double ms[4];
+      amr_dev_read_timing(R->dev, ms);
+      for (int i = 0; i < 4; i++)
+         dev_ms[i] += ms[i];
+      if (check_timestep(R, ts, sums, &R->total_blocks)) {
+         failed = 1;
+         break;
+      }
+      double t4 = amr_now();
+      R->t_cs += t4 - t3;
+
+      if (P->num_refine && !P->uniform_refine) {
+         amr_move_objects(R, delta);
+         if (!(ts % P->refine_freq)) {
+            amr_refine(R, ts);
+            if (R->global_active < R->nb_min)
+               R->nb_min = R->global_active;
+            if (R->global_active > R->nb_max)
+               R->nb_max = R->global_active;
+         }
+      }
+      amr_dev_check(R, amr_dev_sync(R->dev), "sync");
+      R->t_refine_all += amr_now() - t4;
+
+      sim_time += delta;
+      if (P->use_time) {
+         if (P->use_tsteps && ts >= P->num_tsteps)
+            done = 1;
+         delta = amr_calc_time_step(R);
+         if (sim_time >= P->end_time)
+            done = 1;
+      } else if (ts >= P->num_tsteps) {
+         done = 1;
+      }
+   }
+   double ms[4];
+   amr_dev_read_timing(R->dev, ms);
+   for (int i = 0; i < 4; i++)
+      dev_ms[i] += ms[i];
+   R->t_comm = dev_ms[0] * 1e-3;
+   R->t_calc = dev_ms[1] * 1e-3;
+   R->t_cs += dev_ms[2] * 1e-3;
+   if (!failed) {
+      P->end_time = sim_time;
+      P->num_tsteps = ts - 1;
+      R->t_all = amr_now() - t1;
+   }
+   free(sums);
+   if (g_sum_log)
+      fclose(g_sum_log);
+   g_sum_log = NULL;
+   return failed;
+}
+
+/* Summary line of ref/profile.c:1298 plus this build's throughput line. */
+void amr_report(amr_run *R)
+{
+   const amr_params *P = &R->P;
+   double stages = (double) P->num_tsteps * P->stages_per_ts;
+   double cells = (double) P->nx * P->ny * P->nz;
+   double updates = (double) R->total_blocks * cells * P->num_vars;
+   static const char *prec_name[3] = { "f64", "f32", "bf16" };
+   if (P->report_perf & 4) {
+      printf("\n ================ Start report ===================\n\n");
+      printf("          miniAMR hot path on MI355X (gfx950), one rank, one GPU\n\n");
+      printf("block size %d x %d x %d, initial blocks %d x %d x %d, %d variables\n",
+             P->nx, P->ny, P->nz, P->init_block_x, P->init_block_y, P->init_block_z,
+             P->num_vars);
+      printf("stencil %d, storage %s, schedule %s\n", P->stencil,
+             prec_name[P->precision], P->schedule ? "variable-major" : "stage-major");
+      printf("timesteps %d, stages per timestep %d, blocks per stage %.2lf (min %lld max %lld)\n",
+             P->num_tsteps, P->stages_per_ts,
+             stages > 0 ? (double) R->total_blocks / stages : 0.0,
+             (long long) R->nb_min, (long long) R->nb_max);
+      printf("blocks split %lld, blocks consolidated %lld\n",
+             (long long) R->num_refined, (long long) R->num_reformed);
+      printf("time: total %lf init %lf refine %lf; device: exchange %lf stencil %lf checksum %lf\n",
+             R->t_all, R->t_init, R->t_refine_all, R->t_comm, R->t_calc, R->t_cs);
+      if (R->t_all > 0.0)
+         printf("cell-var updates %.6e, %.4lf GCell-var updates/s over the timestep loop\n",
+                updates, updates / R->t_all * 1e-9);
+      printf("\n ================== End report ===================\n");
+   }
+   printf("Summary: ranks %d ts %d time %lf calc %lf comm %lf red %lf refine %lf blocks/ts %lf max_blocks %d\n",
+          1, P->num_tsteps, R->t_all, R->t_calc, R->t_comm, R->t_cs, R->t_refine_all,
+          stages > 0 ? ((double) R->total_blocks) / stages : 0.0, R->local_max_b);
+}
+
+/* Final-state dumps in the formats of the golden-vector harness. */
+int amr_dump_final(amr_run *R)
+{
+   const amr_params *P = &R->P;
+   const char *pc = getenv("AMR_DUMP_CELLS");
+   const char *pb = getenv("AMR_DUMP_BSUMS");
+   if (!pc && !pb)
+      return 0;
+   int gx = P->nx + 2, gy = P->ny + 2, gz = P->nz + 2;
+   size_t slab = (size_t) gx * gy * gz;
+   int na = amr_n_active(R);
+   FILE *fc = pc ? fopen(pc, "wb") : NULL;
+   FILE *fb = pb ? fopen(pb, "wb") : NULL;
+   int32_t hdr[8] = { 0x414d5231, P->nx, P->ny, P->nz, P->num_vars, na, P->num_refine, 0 };
+   if (fc)
+      fwrite(hdr, sizeof(hdr), 1, fc);
+   hdr[0] = 0x414d5332;
+   if (fb)
+      fwrite(hdr, sizeof(hdr), 1, fb);
+   for (int in = 0; in < na; in++) {
+      int slot = R->sorted_slot[in];
+      const amr_block *b = &R->blocks[slot];
+      int64_t number = b->number;
+      int32_t ls[2] = { b->level, slot };
+      amr_dev_check(R, amr_dev_download_block(R->dev, slot, R->cellbuf), "download");
+      if (fc) {
+         fwrite(&number, sizeof(number), 1, fc);
+         fwrite(ls, sizeof(ls), 1, fc);
+         fwrite(R->cellbuf, sizeof(double), slab * P->num_vars, fc);
+      }
+      if (fb) {
+         fwrite(&number, sizeof(number), 1, fb);
+         fwrite(ls, sizeof(ls), 1, fb);
+         for (int v = 0; v < P->num_vars; v++) {
+            double s[2] = { 0.0, 0.0 };
+            const double *a = R->cellbuf + slab * v;
+            for (int i = 0; i < gx; i++)
+               for (int j = 0; j < gy; j++)
+                  for (int k = 0; k < gz; k++) {
+                     int inner = i >= 1 && i <= P->nx && j >= 1 && j <= P->ny &&
+                                 k >= 1 && k <= P->nz;
+                     s[inner ? 0 : 1] += a[((size_t) i * gy + j) * gz + k];
+                  }
+            fwrite(s, sizeof(s), 1, fb);
+         }
+      }
+   }
+   if (fc) fclose(fc);
+   if (fb) fclose(fb);
+   return 0;
+}

@@ -1,0 +1,331 @@
+/*
+ * mesh.c -- block storage, initial mesh, sorted list and exchange plan.
+ *
+ * Initial mesh: ref/init.c:38-683 for one rank with the RCB/Morton layout
+ * (lb_method < 2): blocks numbered x-fastest, slot == number, level 0,
+ * reflective faces on the domain boundary, interior values drawn from libc
+ * rand() in the reference's order (block, var, i, j, k) so that the same
+ * binary on the same libc produces the same field.
+ *
+ * Sorted list: ref/block.c:610-657 -- active blocks grouped by level, each
+ * level in descending block number.  Its order fixes the checksum
+ * accumulation order and the iteration order of refinement passes.
+ *
+ * Exchange plan: the on-rank part of comm() (ref/comm.c:159-203) visits,
+ * per direction, every active block's two faces and performs each
+ * same-level, coarse/fine or boundary transfer once.  Within one direction
+ * every transfer is a gather from cells that no transfer of that direction
+ * writes, so the plan is simply the set of ghost planes each block
+ * receives; it is rebuilt only when the mesh changes.
+ */
+#include <stdarg.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#include <time.h>
+
+#include "amr.h"
+
+void amr_fatal(const char *fmt, ...)
+{
+   va_list ap;
+   va_start(ap, fmt);
+   vprintf(fmt, ap);
+   va_end(ap);
+   fflush(stdout);
+   exit(-1);
+}
+
+void *amr_xmalloc(size_t bytes)
+{
+   void *p = calloc(1, bytes ? bytes : 1);
+   if (!p)
+      amr_fatal("NULL pointer from malloc call (%zu bytes)\n", bytes);
+   return p;
+}
+
+void amr_vec_push(amr_vec *v, const void *elem)
+{
+   if (v->n == v->cap) {
+      v->cap = v->cap ? 2 * v->cap : 256;
+      v->v = realloc(v->v, (size_t) v->cap * v->elem);
+      if (!v->v)
+         amr_fatal("out of host memory for job list\n");
+   }
+   memcpy((char *) v->v + (size_t) v->n * v->elem, elem, v->elem);
+   v->n++;
+}
+
+double amr_now(void)
+{
+   struct timespec ts;
+   clock_gettime(CLOCK_MONOTONIC, &ts);
+   return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+int amr_dev_check(amr_run *R, int rc, const char *what)
+{
+   if (rc != AMR_OK)
+      amr_fatal("ERROR: device %s failed (%d): %s\n", what, rc,
+                amr_dev_last_error(R->dev));
+   return rc;
+}
+
+int amr_allocate(amr_run *R)
+{
+   const amr_params *P = &R->P;
+   int nl = P->num_refine + 1;
+
+   R->blocks = (amr_block *) amr_xmalloc(sizeof(amr_block) * P->max_num_blocks);
+   for (int n = 0; n < P->max_num_blocks; n++)
+      R->blocks[n].number = -1;
+   R->max_num_parents = P->max_num_blocks;   /* same guess as ref/main.c:457 */
+   R->parents = (amr_parent *) amr_xmalloc(sizeof(amr_parent) * R->max_num_parents);
+   for (int n = 0; n < R->max_num_parents; n++)
+      R->parents[n].number = -1;
+   R->sorted_number = (int64_t *) amr_xmalloc(sizeof(int64_t) * (P->max_num_blocks + 1));
+   R->sorted_slot = (int *) amr_xmalloc(sizeof(int) * (P->max_num_blocks + 1));
+   R->sorted_index = (int *) amr_xmalloc(sizeof(int) * (nl + 1));
+   R->num_blocks = (int64_t *) amr_xmalloc(sizeof(int64_t) * nl);
+   R->block_start = (int64_t *) amr_xmalloc(sizeof(int64_t) * nl);
+   R->p8 = (int64_t *) amr_xmalloc(sizeof(int64_t) * (nl + 1));
+   R->p2 = (int *) amr_xmalloc(sizeof(int) * (nl + 1));
+   R->grid_sum = (double *) amr_xmalloc(sizeof(double) * P->num_vars);
+   R->scratch_i = (int *) amr_xmalloc(sizeof(int) * P->max_num_blocks);
+   R->cellbuf = (double *) amr_xmalloc(sizeof(double) * (size_t) P->num_vars *
+                                       (P->nx + 2) * (P->ny + 2) * (P->nz + 2));
+   R->split_jobs.elem = sizeof(amr_split_job);
+   R->merge_jobs.elem = sizeof(amr_merge_job);
+   R->face_jobs.elem = sizeof(amr_face_job);
+
+   amr_dev_config cfg;
+   cfg.nx = P->nx; cfg.ny = P->ny; cfg.nz = P->nz;
+   cfg.num_vars = P->num_vars;
+   cfg.max_blocks = P->max_num_blocks;
+   cfg.stencil = P->stencil;
+   cfg.precision = P->precision;
+   cfg.device = P->device;
+   int rc = amr_dev_create(&cfg, &R->dev);
+   if (rc != AMR_OK)
+      amr_fatal("ERROR: could not create the device context (%d): %s\n", rc,
+                R->dev ? amr_dev_last_error(R->dev) : "no context");
+   return 0;
+}
+
+void amr_free(amr_run *R)
+{
+   if (R->dev)
+      amr_dev_destroy(R->dev);
+   free(R->blocks); free(R->parents); free(R->sorted_number); free(R->sorted_slot);
+   free(R->sorted_index); free(R->num_blocks); free(R->block_start);
+   free(R->p8); free(R->p2); free(R->grid_sum); free(R->scratch_i); free(R->cellbuf);
+   free(R->split_jobs.v); free(R->merge_jobs.v); free(R->face_jobs.v);
+   free(R->P.objects);
+   memset(R, 0, sizeof(*R));
+}
+
+int amr_n_active(const amr_run *R)
+{
+   return R->sorted_index[R->P.num_refine + 1];
+}
+
+/* Insert before the first entry of the level with a smaller number
+ * (ref/block.c:610-626). */
+void amr_sorted_add(amr_run *R, int slot, int64_t number, int level)
+{
+   int last = R->P.num_refine + 1;
+   int end = R->sorted_index[last];
+   int i = R->sorted_index[level];
+   while (i < R->sorted_index[level + 1] && !(number > R->sorted_number[i]))
+      i++;
+   memmove(&R->sorted_number[i + 1], &R->sorted_number[i], sizeof(int64_t) * (end - i));
+   memmove(&R->sorted_slot[i + 1], &R->sorted_slot[i], sizeof(int) * (end - i));
+   R->sorted_number[i] = number;
+   R->sorted_slot[i] = slot;
+   for (int l = level + 1; l <= last; l++)
+      R->sorted_index[l]++;
+}
+
+/* ref/block.c:628-646 */
+void amr_sorted_del(amr_run *R, int64_t number, int level)
+{
+   int last = R->P.num_refine + 1;
+   int i = R->sorted_index[level];
+   while (i < R->sorted_index[level + 1] && R->sorted_number[i] != number)
+      i++;
+   if (i >= R->sorted_index[level + 1])
+      amr_fatal("ERROR: del_sorted_list on %d - number %lld not found l %d\n",
+                0, (long long) number, level);
+   int end = R->sorted_index[last];
+   memmove(&R->sorted_number[i], &R->sorted_number[i + 1], sizeof(int64_t) * (end - i - 1));
+   memmove(&R->sorted_slot[i], &R->sorted_slot[i + 1], sizeof(int) * (end - i - 1));
+   for (int l = level + 1; l <= last; l++)
+      R->sorted_index[l]--;
+}
+
+/* Initial mesh and values (ref/init.c:38-683, one rank, lb_method < 2). */
+int amr_init(amr_run *R)
+{
+   amr_params *P = &R->P;
+   int nr = P->num_refine;
+
+   R->tol = pow(10.0, (double) -P->error_tol);
+   R->p2[0] = 1;
+   R->p8[0] = 1;
+   for (int i = 0; i <= nr; i++) {
+      R->p8[i + 1] = R->p8[i] * 8;
+      R->p2[i + 1] = R->p2[i] * 2;
+      R->sorted_index[i] = 0;
+   }
+   R->sorted_index[nr + 1] = 0;
+
+   int n0 = P->init_block_x * P->init_block_y * P->init_block_z;
+   R->local_max_b = n0;
+   R->block_start[0] = 0;
+   R->num_blocks[0] = n0;
+   int64_t num = n0;
+   for (int i = 1; i <= nr; i++) {
+      R->block_start[i] = R->block_start[i - 1] + num;
+      num *= 8;
+      R->num_blocks[i] = 0;
+   }
+   R->init_x = P->init_block_x;
+   R->init_y = P->init_block_y;
+   R->init_z = P->init_block_z;
+
+   R->num_active = R->max_active_block = n0;
+   R->global_active = n0;
+   R->num_parents = R->max_active_parent = 0;
+   int size = R->p2[nr + 1];
+   R->mesh_size[0] = R->init_x * size;
+   R->mesh_size[1] = R->init_y * size;
+   R->mesh_size[2] = R->init_z * size;
+
+   const int gx = P->nx + 2, gy = P->ny + 2, gz = P->nz + 2;
+   const size_t slab = (size_t) gx * gy * gz;
+   int n = 0;
+   for (int k = 0; k < P->init_block_z; k++)
+      for (int j = 0; j < P->init_block_y; j++)
+         for (int i = 0; i < P->init_block_x; i++, n++) {
+            amr_block *b = &R->blocks[n];
+            int idx[3] = { i, j, k };
+            int cnt[3] = { P->init_block_x, P->init_block_y, P->init_block_z };
+            int stride[3] = { 1, P->init_block_x, P->init_block_x * P->init_block_y };
+            b->level = 0;
+            b->number = n;
+            b->refine = 0;
+            b->parent = -1;
+            b->child_number = 0;
+            b->cen[0] = i * size + size / 2;
+            b->cen[1] = j * size + size / 2;
+            b->cen[2] = k * size + size / 2;
+            amr_sorted_add(R, n, n, 0);
+            /* zero the whole slot, then the interior from rand() */
+            memset(R->cellbuf, 0, sizeof(double) * slab * P->num_vars);
+            for (int v = 0; v < P->num_vars; v++) {
+               double *a = R->cellbuf + slab * v;
+               for (int x = 1; x <= P->nx; x++)
+                  for (int y = 1; y <= P->ny; y++)
+                     for (int z = 1; z <= P->nz; z++)
+                        a[((size_t) x * gy + y) * gz + z] =
+                           ((double) rand()) / ((double) RAND_MAX);
+            }
+            amr_dev_check(R, amr_dev_upload_block(R->dev, n, R->cellbuf), "upload");
+            memset(b->nei, 0, sizeof(b->nei));
+            for (int d = 0; d < 3; d++) {
+               /* low face 2d, high face 2d+1 */
+               if (idx[d] == 0) {
+                  b->nei_level[2 * d] = AMR_BOUNDARY;
+                  b->nei[2 * d][0][0] = 0;
+               } else {
+                  b->nei_level[2 * d] = 0;
+                  b->nei[2 * d][0][0] = n - stride[d];
+               }
+               if (idx[d] == cnt[d] - 1) {
+                  b->nei_level[2 * d + 1] = AMR_BOUNDARY;
+                  b->nei[2 * d + 1][0][0] = 0;
+               } else {
+                  b->nei_level[2 * d + 1] = 0;
+                  b->nei[2 * d + 1][0][0] = n + stride[d];
+               }
+            }
+         }
+   R->mesh_dirty = 1;
+   amr_sync_mesh(R);
+   for (int v = 0; v < P->num_vars; v++)
+      R->grid_sum[v] = amr_checksum(R, v);
+   return 0;
+}
+
+static int opposite(int f) { return f ^ 1; }
+
+/* Ghost planes each active block receives, per direction
+ * (ref/comm.c:159-203 with on_proc_comm, on_proc_comm_diff, apply_bc). */
+int amr_build_exchange_plan(amr_run *R)
+{
+   amr_vec *v = &R->face_jobs;
+   int32_t count[3] = { 0, 0, 0 };
+   int na = amr_n_active(R);
+   v->n = 0;
+   for (int dir = 0; dir < 3; dir++) {
+      int before = v->n;
+      for (int in = 0; in < na; in++) {
+         int s = R->sorted_slot[in];
+         const amr_block *b = &R->blocks[s];
+         for (int f = 2 * dir; f < 2 * dir + 2; f++) {
+            amr_face_job j;
+            memset(&j, 0, sizeof(j));
+            j.face = f;
+            j.dst = s;
+            if (b->nei_level[f] == b->level) {
+               j.kind = AMR_FACE_SAME;
+               j.src = b->nei[f][0][0];
+               amr_vec_push(v, &j);
+            } else if (b->nei_level[f] == b->level + 1) {
+               /* coarse side: one restriction per finer neighbour */
+               for (int iq = 0; iq < 2; iq++)
+                  for (int jq = 0; jq < 2; jq++) {
+                     j.kind = AMR_FACE_RESTRICT;
+                     j.src = b->nei[f][iq][jq];
+                     j.iq = iq;
+                     j.jq = jq;
+                     amr_vec_push(v, &j);
+                  }
+            } else if (b->nei_level[f] == b->level - 1) {
+               /* fine side: find which quarter of the coarse face we are */
+               int m = b->nei[f][0][0], g = opposite(f), found = 0;
+               for (int iq = 0; iq < 2 && !found; iq++)
+                  for (int jq = 0; jq < 2 && !found; jq++)
+                     if (R->blocks[m].nei[g][iq][jq] == s) {
+                        j.kind = AMR_FACE_PROLONG;
+                        j.src = m;
+                        j.iq = iq;
+                        j.jq = jq;
+                        amr_vec_push(v, &j);
+                        found = 1;
+                     }
+            } else if (b->nei_level[f] == AMR_BOUNDARY) {
+               j.kind = AMR_FACE_BC;
+               j.src = s;
+               amr_vec_push(v, &j);
+            } else {
+               amr_fatal("ERROR: misconnected block\n");
+            }
+         }
+      }
+      count[dir] = v->n - before;
+   }
+   return amr_dev_check(R, amr_dev_set_mesh(R->dev, R->sorted_slot, na,
+                                            (const amr_face_job *) v->v, count),
+                        "set_mesh");
+}
+
+/* Push pending split/consolidate data work and a changed mesh to the device. */
+int amr_sync_mesh(amr_run *R)
+{
+   if (R->mesh_dirty) {
+      amr_build_exchange_plan(R);
+      R->mesh_dirty = 0;
+   }
+   return 0;
+}

@@ -1,0 +1,360 @@
+/*
+ * amr_cpu.c -- CPU oracle for the miniAMR hot path (TEST INFRASTRUCTURE).
+ *
+ * A plain-C implementation of include/amr_dev.h that restates the
+ * reference's arithmetic in the reference's exact operation order, FP64
+ * only.  It is linked into oracle/_build/miniAMR_oracle.x (with the
+ * product's host driver) and built as oracle/_build/liboracle_dev.so for
+ * ctypes.  Only tests, __graft_entry__.smoke() and bench.py's CPU-baseline
+ * leg use it, as the checker; the product never links it.
+ *
+ * Functions and the reference code they restate:
+ *   stencil_7 / stencil_27   ref/stencil.c:82-101 / 102-144
+ *   fill_same                ref/comm.c:1473-1534 (on_proc_comm, code 0)
+ *   fill_prolong / restrict  ref/comm.c:1597-1688 (on_proc_comm_diff, code 0)
+ *   fill_bc                  ref/comm.c:1911-1965 (apply_bc)
+ *   check_sum                ref/check_sum.c:36-65
+ *   split                    ref/block.c:161-173
+ *   consolidate              ref/block.c:411-431
+ * Parity: pinned by the golden fixtures of tests/golden (checksums at
+ * %.17g and full cell dumps of the reference on the same inputs).
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdint.h>
+
+#include "amr_dev.h"
+
+struct amr_dev {
+   amr_dev_config c;
+   int gx, gy, gz;
+   size_t slab;                 /* doubles per (var, slot) */
+   double *pool;                /* [var][slot][gx][gy][gz] */
+   int32_t *sorted;
+   int n_active;
+   amr_face_job *jobs;
+   int32_t count[3];
+   double *sums;
+   int sum_rows;
+   double *work;                /* one slab, stencil scratch */
+   char err[256];
+};
+
+#define A(d, v, s, i, j, k) \
+   ((d)->pool[((size_t) (v) * (d)->c.max_blocks + (s)) * (d)->slab + \
+              ((size_t) (i) * (d)->gy + (j)) * (d)->gz + (k)])
+
+static int fail(amr_dev *d, int rc, const char *msg)
+{
+   if (d)
+      snprintf(d->err, sizeof(d->err), "%s", msg);
+   return rc;
+}
+
+int amr_dev_create(const amr_dev_config *cfg, amr_dev **out)
+{
+   *out = NULL;
+   if (!cfg || cfg->nx < 2 || cfg->ny < 2 || cfg->nz < 2 || (cfg->nx | cfg->ny | cfg->nz) & 1 ||
+       cfg->num_vars < 1 || cfg->max_blocks < 1 ||
+       (cfg->stencil != 7 && cfg->stencil != 27) || cfg->precision != AMR_PREC_F64)
+      return AMR_ERR_ARG;
+   amr_dev *d = (amr_dev *) calloc(1, sizeof(amr_dev));
+   d->c = *cfg;
+   d->gx = cfg->nx + 2; d->gy = cfg->ny + 2; d->gz = cfg->nz + 2;
+   d->slab = (size_t) d->gx * d->gy * d->gz;
+   d->pool = (double *) calloc(d->slab * cfg->num_vars * cfg->max_blocks, sizeof(double));
+   d->work = (double *) calloc(d->slab, sizeof(double));
+   d->sorted = (int32_t *) calloc(cfg->max_blocks, sizeof(int32_t));
+   if (!d->pool || !d->work || !d->sorted) {
+      free(d->pool); free(d->work); free(d->sorted); free(d);
+      return AMR_ERR_NOMEM;
+   }
+   *out = d;
+   return AMR_OK;
+}
+
+int amr_dev_destroy(amr_dev *d)
+{
+   if (!d) return AMR_OK;
+   free(d->pool); free(d->work); free(d->sorted); free(d->jobs); free(d->sums);
+   free(d);
+   return AMR_OK;
+}
+
+const char *amr_dev_last_error(const amr_dev *d) { return d ? d->err : "no context"; }
+
+int amr_dev_upload_block(amr_dev *d, int slot, const double *cells)
+{
+   if (slot < 0 || slot >= d->c.max_blocks)
+      return fail(d, AMR_ERR_ARG, "slot out of range");
+   for (int v = 0; v < d->c.num_vars; v++)
+      memcpy(&A(d, v, slot, 0, 0, 0), cells + d->slab * v, sizeof(double) * d->slab);
+   return AMR_OK;
+}
+
+int amr_dev_download_block(amr_dev *d, int slot, double *cells)
+{
+   if (slot < 0 || slot >= d->c.max_blocks)
+      return fail(d, AMR_ERR_ARG, "slot out of range");
+   for (int v = 0; v < d->c.num_vars; v++)
+      memcpy(cells + d->slab * v, &A(d, v, slot, 0, 0, 0), sizeof(double) * d->slab);
+   return AMR_OK;
+}
+
+int amr_dev_set_mesh(amr_dev *d, const int32_t *sorted, int n_active,
+                     const amr_face_job *jobs, const int32_t count[3])
+{
+   int nj = count[0] + count[1] + count[2];
+   memcpy(d->sorted, sorted, sizeof(int32_t) * n_active);
+   d->n_active = n_active;
+   free(d->jobs);
+   d->jobs = (amr_face_job *) malloc(sizeof(amr_face_job) * (nj ? nj : 1));
+   memcpy(d->jobs, jobs, sizeof(amr_face_job) * nj);
+   memcpy(d->count, count, sizeof(d->count));
+   return AMR_OK;
+}
+
+/* n of axis a (0 = x, 1 = y, 2 = z) */
+static int nsz(const amr_dev *d, int a) { return a == 0 ? d->c.nx : a == 1 ? d->c.ny : d->c.nz; }
+
+/* Cell reference by axis-ordered coordinates: dir-axis index p and the
+ * two in-plane indices (a slower, b faster) of a face of direction dir. */
+static double *cell(amr_dev *d, int v, int s, int dir, int p, int a, int b)
+{
+   if (dir == 0) return &A(d, v, s, p, a, b);   /* x face: (j, k) */
+   if (dir == 1) return &A(d, v, s, a, p, b);   /* y face: (i, k) */
+   return &A(d, v, s, a, b, p);                 /* z face: (i, j) */
+}
+
+/* in-plane axes of a face of direction dir: slower, faster */
+static void plane_axes(int dir, int *sa, int *sb)
+{
+   *sa = dir == 0 ? 1 : 0;
+   *sb = dir == 2 ? 1 : 2;
+}
+
+/* Extended in-plane ranges for the 27-point stencil (ref/comm.c:1496-1533,
+ * 1941-1964): y faces extend x, z faces extend x and y, BC extends both. */
+static void ranges(amr_dev *d, const amr_face_job *j, int *a0, int *a1, int *b0, int *b1)
+{
+   int dir = j->face / 2, sa, sb;
+   plane_axes(dir, &sa, &sb);
+   int ext_a = 0, ext_b = 0;
+   if (d->c.stencil == 27) {
+      if (j->kind == AMR_FACE_BC) ext_a = ext_b = 1;
+      else if (dir == 1) ext_a = 1;
+      else if (dir == 2) ext_a = ext_b = 1;
+   }
+   *a0 = ext_a ? 0 : 1; *a1 = nsz(d, sa) + (ext_a ? 1 : 0);
+   *b0 = ext_b ? 0 : 1; *b1 = nsz(d, sb) + (ext_b ? 1 : 0);
+}
+
+static void run_job(amr_dev *d, const amr_face_job *j, int v)
+{
+   int dir = j->face / 2, hi = j->face % 2, sa, sb;
+   int n = nsz(d, dir);
+   int ghost = hi ? n + 1 : 0;     /* plane written */
+   int src_pl = hi ? 1 : n;        /* neighbour's plane read (SAME/PROLONG/RESTRICT) */
+   plane_axes(dir, &sa, &sb);
+   int ha = nsz(d, sa) / 2, hb = nsz(d, sb) / 2;
+   int a0, a1, b0, b1;
+   switch (j->kind) {
+   case AMR_FACE_SAME:
+      ranges(d, j, &a0, &a1, &b0, &b1);
+      for (int a = a0; a <= a1; a++)
+         for (int b = b0; b <= b1; b++)
+            *cell(d, v, j->dst, dir, ghost, a, b) = *cell(d, v, j->src, dir, src_pl, a, b);
+      break;
+   case AMR_FACE_BC:
+      ranges(d, j, &a0, &a1, &b0, &b1);
+      for (int a = a0; a <= a1; a++)
+         for (int b = b0; b <= b1; b++)
+            *cell(d, v, j->dst, dir, ghost, a, b) = *cell(d, v, j->dst, dir, hi ? n : 1, a, b);
+      break;
+   case AMR_FACE_PROLONG: {
+      /* fine ghost quad <- coarse cell / 4; a offset jq*ha, b offset iq*hb */
+      int ao = j->jq * ha, bo = j->iq * hb;
+      for (int a = 1; a <= ha; a++)
+         for (int b = 1; b <= hb; b++) {
+            double q = *cell(d, v, j->src, dir, src_pl, a + ao, b + bo) / 4.0;
+            *cell(d, v, j->dst, dir, ghost, 2 * a - 1, 2 * b - 1) = q;
+            *cell(d, v, j->dst, dir, ghost, 2 * a - 1, 2 * b) = q;
+            *cell(d, v, j->dst, dir, ghost, 2 * a, 2 * b - 1) = q;
+            *cell(d, v, j->dst, dir, ghost, 2 * a, 2 * b) = q;
+         }
+      break;
+   }
+   case AMR_FACE_RESTRICT: {
+      int ao = j->jq * ha, bo = j->iq * hb;
+      for (int a = 1; a <= ha; a++)
+         for (int b = 1; b <= hb; b++)
+            *cell(d, v, j->dst, dir, ghost, a + ao, b + bo) =
+               *cell(d, v, j->src, dir, src_pl, 2 * a - 1, 2 * b - 1) +
+               *cell(d, v, j->src, dir, src_pl, 2 * a - 1, 2 * b) +
+               *cell(d, v, j->src, dir, src_pl, 2 * a, 2 * b - 1) +
+               *cell(d, v, j->src, dir, src_pl, 2 * a, 2 * b);
+      break;
+   }
+   }
+}
+
+int amr_dev_comm(amr_dev *d, int start, int num, int stage, int permute)
+{
+   static const int perm[6][3] = { {0, 1, 2}, {1, 2, 0}, {2, 0, 1},
+                                   {0, 2, 1}, {1, 0, 2}, {2, 1, 0} };
+   if (start < 0 || num < 0 || start + num > d->c.num_vars)
+      return fail(d, AMR_ERR_ARG, "variable range");
+   int off[3] = { 0, d->count[0], d->count[0] + d->count[1] };
+   for (int o = 0; o < 3; o++) {
+      int dir = permute ? perm[stage % 6][o] : o;
+      for (int v = start; v < start + num; v++)
+         for (int t = 0; t < d->count[dir]; t++)
+            run_job(d, &d->jobs[off[dir] + t], v);
+   }
+   return AMR_OK;
+}
+
+int amr_dev_stencil(amr_dev *d, int var)
+{
+   int nx = d->c.nx, ny = d->c.ny, nz = d->c.nz;
+   double *w = d->work;
+#define W(i, j, k) w[((size_t) (i) * d->gy + (j)) * d->gz + (k)]
+   for (int in = 0; in < d->n_active; in++) {
+      int s = d->sorted[in];
+      if (d->c.stencil == 7) {
+         for (int i = 1; i <= nx; i++)
+            for (int j = 1; j <= ny; j++)
+               for (int k = 1; k <= nz; k++)
+                  W(i, j, k) = (A(d, var, s, i - 1, j, k) + A(d, var, s, i, j - 1, k) +
+                                A(d, var, s, i, j, k - 1) + A(d, var, s, i, j, k) +
+                                A(d, var, s, i, j, k + 1) + A(d, var, s, i, j + 1, k) +
+                                A(d, var, s, i + 1, j, k)) / 7.0;
+      } else {
+         for (int i = 1; i <= nx; i++)
+            for (int j = 1; j <= ny; j++)
+               for (int k = 1; k <= nz; k++) {
+                  double sp[3];
+                  for (int p = 0; p < 3; p++) {
+                     int ii = i - 1 + p;
+                     double acc = A(d, var, s, ii, j - 1, k - 1);
+                     acc = acc + A(d, var, s, ii, j - 1, k);
+                     acc = acc + A(d, var, s, ii, j - 1, k + 1);
+                     acc = acc + A(d, var, s, ii, j, k - 1);
+                     acc = acc + A(d, var, s, ii, j, k);
+                     acc = acc + A(d, var, s, ii, j, k + 1);
+                     acc = acc + A(d, var, s, ii, j + 1, k - 1);
+                     acc = acc + A(d, var, s, ii, j + 1, k);
+                     acc = acc + A(d, var, s, ii, j + 1, k + 1);
+                     sp[p] = acc;
+                  }
+                  W(i, j, k) = (sp[0] + sp[1] + sp[2]) / 27.0;
+               }
+      }
+      for (int i = 1; i <= nx; i++)
+         for (int j = 1; j <= ny; j++)
+            for (int k = 1; k <= nz; k++)
+               A(d, var, s, i, j, k) = W(i, j, k);
+   }
+#undef W
+   return AMR_OK;
+}
+
+int amr_dev_check_sum(amr_dev *d, int var, double *out)
+{
+   double sum = 0.0;
+   for (int in = 0; in < d->n_active; in++) {
+      int s = d->sorted[in];
+      double bs = 0.0;
+      for (int i = 1; i <= d->c.nx; i++)
+         for (int j = 1; j <= d->c.ny; j++)
+            for (int k = 1; k <= d->c.nz; k++)
+               bs += A(d, var, s, i, j, k);
+      sum += bs;
+   }
+   *out = sum;
+   return AMR_OK;
+}
+
+int amr_dev_split(amr_dev *d, const amr_split_job *jobs, int n)
+{
+   int hx = d->c.nx / 2, hy = d->c.ny / 2, hz = d->c.nz / 2;
+   for (int t = 0; t < n; t++) {
+      const amr_split_job *j = &jobs[t];
+      int o = j->octant;
+      int i1 = (o % 2) * hx, j1 = ((o / 2) % 2) * hy, k1 = (o / 4) * hz;
+      for (int v = 0; v < d->c.num_vars; v++)
+         for (int i = 1; i <= hx; i++)
+            for (int jj = 1; jj <= hy; jj++)
+               for (int k = 1; k <= hz; k++) {
+                  double q = A(d, v, j->parent, i + i1, jj + j1, k + k1) / 8.0;
+                  for (int di = 0; di < 2; di++)
+                     for (int dj = 0; dj < 2; dj++)
+                        for (int dk = 0; dk < 2; dk++)
+                           A(d, v, j->child, 2 * i - 1 + di, 2 * jj - 1 + dj, 2 * k - 1 + dk) = q;
+               }
+   }
+   return AMR_OK;
+}
+
+int amr_dev_consolidate(amr_dev *d, const amr_merge_job *jobs, int n)
+{
+   int hx = d->c.nx / 2, hy = d->c.ny / 2, hz = d->c.nz / 2;
+   for (int t = 0; t < n; t++) {
+      const amr_merge_job *j = &jobs[t];
+      int o = j->octant, c = j->child;
+      int i1 = (o % 2) * hx, j1 = ((o / 2) % 2) * hy, k1 = (o / 4) * hz;
+      for (int v = 0; v < d->c.num_vars; v++)
+         for (int i = 1; i <= hx; i++)
+            for (int jj = 1; jj <= hy; jj++)
+               for (int k = 1; k <= hz; k++) {
+                  int i2 = 2 * i - 1, j2 = 2 * jj - 1, k2 = 2 * k - 1;
+                  A(d, v, j->dst, i + i1, jj + j1, k + k1) =
+                     A(d, v, c, i2, j2, k2) + A(d, v, c, i2 + 1, j2, k2) +
+                     A(d, v, c, i2, j2 + 1, k2) + A(d, v, c, i2 + 1, j2 + 1, k2) +
+                     A(d, v, c, i2, j2, k2 + 1) + A(d, v, c, i2 + 1, j2, k2 + 1) +
+                     A(d, v, c, i2, j2 + 1, k2 + 1) + A(d, v, c, i2 + 1, j2 + 1, k2 + 1);
+               }
+   }
+   return AMR_OK;
+}
+
+int amr_dev_reserve_sums(amr_dev *d, int n_rows)
+{
+   free(d->sums);
+   d->sums = (double *) calloc((size_t) (n_rows > 0 ? n_rows : 1) * d->c.num_vars, sizeof(double));
+   d->sum_rows = n_rows;
+   return AMR_OK;
+}
+
+int amr_dev_read_sums(amr_dev *d, int row0, int n_rows, double *out)
+{
+   if (row0 < 0 || row0 + n_rows > d->sum_rows)
+      return fail(d, AMR_ERR_ARG, "sum rows");
+   memcpy(out, d->sums + (size_t) row0 * d->c.num_vars,
+          sizeof(double) * (size_t) n_rows * d->c.num_vars);
+   return AMR_OK;
+}
+
+int amr_dev_stage(amr_dev *d, int start, int num, int stage, int permute, int sum_row)
+{
+   int rc = amr_dev_comm(d, start, num, stage, permute);
+   if (rc) return rc;
+   if (sum_row >= d->sum_rows)
+      return fail(d, AMR_ERR_ARG, "sum row");
+   for (int v = start; v < start + num; v++) {
+      amr_dev_stencil(d, v);
+      if (sum_row >= 0)
+         amr_dev_check_sum(d, v, &d->sums[(size_t) sum_row * d->c.num_vars + v]);
+   }
+   return AMR_OK;
+}
+
+int amr_dev_sync(amr_dev *d) { (void) d; return AMR_OK; }
+int amr_dev_set_timing(amr_dev *d, int on) { (void) d; (void) on; return AMR_OK; }
+int amr_dev_read_timing(amr_dev *d, double ms[4])
+{
+   (void) d;
+   ms[0] = ms[1] = ms[2] = ms[3] = 0.0;
+   return AMR_OK;
+}
