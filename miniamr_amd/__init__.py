@@ -1,0 +1,205 @@
+"""miniAMR hot path on MI355X (gfx950): Python view of the C ABI.
+
+The product is two in-tree artefacts built by ``make -C miniamr_amd``:
+
+* ``lib/libminiamr_dev.so`` -- HIP kernels behind the C ABI declared in
+  ``include/amr_dev.h`` (stencil, ghost exchange, checksum, split and
+  consolidate, all resident in device memory);
+* ``bin/miniAMR.x`` -- the drop-in driver: the reference's command line,
+  block/object model and refinement decisions in C, calling the library.
+
+This module loads the library with ctypes (for tests, the benchmark and
+Python callers) and runs the driver.  There is no CPU fallback: if the
+library is missing or no gfx950 device is present, calls raise.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(PKG)
+LIB_PATH = os.path.join(PKG, "lib", "libminiamr_dev.so")
+EXE_PATH = os.path.join(PKG, "bin", "miniAMR.x")
+INCLUDE = os.path.join(REPO, "include", "amr_dev.h")
+
+AMR_OK = 0
+PREC_F64, PREC_F32, PREC_BF16 = 0, 1, 2
+FACE_SAME, FACE_PROLONG, FACE_RESTRICT, FACE_BC = 0, 1, 2, 3
+
+# Every symbol include/amr_dev.h declares.
+ABI_SYMBOLS = [
+    "amr_dev_create", "amr_dev_destroy", "amr_dev_last_error", "amr_dev_upload_block",
+    "amr_dev_download_block", "amr_dev_set_mesh", "amr_dev_comm", "amr_dev_stencil",
+    "amr_dev_check_sum", "amr_dev_split", "amr_dev_consolidate", "amr_dev_reserve_sums",
+    "amr_dev_read_sums", "amr_dev_stage", "amr_dev_sync", "amr_dev_set_timing",
+    "amr_dev_read_timing",
+]
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("num_vars", ctypes.c_int), ("max_blocks", ctypes.c_int),
+                ("stencil", ctypes.c_int), ("precision", ctypes.c_int),
+                ("device", ctypes.c_int)]
+
+
+FACE_JOB = np.dtype([("kind", "<i4"), ("face", "<i4"), ("dst", "<i4"), ("src", "<i4"),
+                     ("iq", "<i4"), ("jq", "<i4")])
+SPLIT_JOB = np.dtype([("parent", "<i4"), ("child", "<i4"), ("octant", "<i4"), ("pad", "<i4")])
+MERGE_JOB = np.dtype([("dst", "<i4"), ("child", "<i4"), ("octant", "<i4"), ("pad", "<i4")])
+
+
+class AmrError(RuntimeError):
+    pass
+
+
+def load_library(path=LIB_PATH):
+    """Load an implementation of amr_dev.h (the HIP library by default)."""
+    if not os.path.exists(path):
+        raise AmrError("%s is not built: run `make -C miniamr_amd` (or "
+                       "__graft_entry__.build())" % path)
+    lib = ctypes.CDLL(path)
+    vp, ip, i32p, dp = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int32), \
+        ctypes.POINTER(ctypes.c_double)
+    sig = {
+        "amr_dev_create": (ip, [ctypes.POINTER(Config), ctypes.POINTER(vp)]),
+        "amr_dev_destroy": (ip, [vp]),
+        "amr_dev_last_error": (ctypes.c_char_p, [vp]),
+        "amr_dev_upload_block": (ip, [vp, ip, dp]),
+        "amr_dev_download_block": (ip, [vp, ip, dp]),
+        "amr_dev_set_mesh": (ip, [vp, i32p, ip, vp, i32p]),
+        "amr_dev_comm": (ip, [vp, ip, ip, ip, ip]),
+        "amr_dev_stencil": (ip, [vp, ip]),
+        "amr_dev_check_sum": (ip, [vp, ip, dp]),
+        "amr_dev_split": (ip, [vp, vp, ip]),
+        "amr_dev_consolidate": (ip, [vp, vp, ip]),
+        "amr_dev_reserve_sums": (ip, [vp, ip]),
+        "amr_dev_read_sums": (ip, [vp, ip, ip, dp]),
+        "amr_dev_stage": (ip, [vp, ip, ip, ip, ip, ip]),
+        "amr_dev_sync": (ip, [vp]),
+        "amr_dev_set_timing": (ip, [vp, ip]),
+        "amr_dev_read_timing": (ip, [vp, dp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+class Device:
+    """One amr_dev context.  Cells are exchanged as float64 arrays of shape
+    (num_vars, nx+2, ny+2, nz+2), the reference's array[var][i][j][k]."""
+
+    def __init__(self, nx, ny, nz, num_vars, max_blocks, stencil=7, precision=PREC_F64,
+                 device=0, lib=None):
+        self.lib = lib if lib is not None else load_library()
+        self.cfg = Config(nx, ny, nz, num_vars, max_blocks, stencil, precision, device)
+        self.shape = (num_vars, nx + 2, ny + 2, nz + 2)
+        h = ctypes.c_void_p()
+        rc = self.lib.amr_dev_create(ctypes.byref(self.cfg), ctypes.byref(h))
+        self.h = h
+        if rc != AMR_OK:
+            msg = self.lib.amr_dev_last_error(h) if h.value else b"no context"
+            self.close()
+            raise AmrError("amr_dev_create failed (%d): %s" % (rc, msg.decode()))
+
+    def _check(self, rc, what):
+        if rc != AMR_OK:
+            raise AmrError("%s failed (%d): %s" % (
+                what, rc, self.lib.amr_dev_last_error(self.h).decode()))
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.amr_dev_destroy(self.h)
+        self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, slot, cells):
+        a = np.ascontiguousarray(cells, dtype=np.float64).reshape(self.shape)
+        self._check(self.lib.amr_dev_upload_block(self.h, slot, _ptr(a, ctypes.c_double)),
+                    "upload")
+
+    def download(self, slot):
+        a = np.empty(self.shape, dtype=np.float64)
+        self._check(self.lib.amr_dev_download_block(self.h, slot, _ptr(a, ctypes.c_double)),
+                    "download")
+        return a
+
+    def set_mesh(self, sorted_slots, jobs_by_dir):
+        s = np.ascontiguousarray(sorted_slots, dtype=np.int32)
+        parts = [np.asarray(j, dtype=FACE_JOB) for j in jobs_by_dir]
+        jobs = np.ascontiguousarray(np.concatenate(parts) if parts else np.zeros(0, FACE_JOB))
+        cnt = np.array([len(p) for p in parts], dtype=np.int32)
+        self._check(self.lib.amr_dev_set_mesh(self.h, _ptr(s, ctypes.c_int32), len(s),
+                                              jobs.ctypes.data_as(ctypes.c_void_p),
+                                              _ptr(cnt, ctypes.c_int32)), "set_mesh")
+
+    def comm(self, start, num, stage=0, permute=False):
+        self._check(self.lib.amr_dev_comm(self.h, start, num, stage, int(permute)), "comm")
+
+    def stencil(self, var):
+        self._check(self.lib.amr_dev_stencil(self.h, var), "stencil")
+
+    def check_sum(self, var):
+        out = ctypes.c_double()
+        self._check(self.lib.amr_dev_check_sum(self.h, var, ctypes.byref(out)), "check_sum")
+        return out.value
+
+    def split(self, jobs):
+        j = np.ascontiguousarray(jobs, dtype=SPLIT_JOB)
+        self._check(self.lib.amr_dev_split(self.h, j.ctypes.data_as(ctypes.c_void_p), len(j)),
+                    "split")
+
+    def consolidate(self, jobs):
+        j = np.ascontiguousarray(jobs, dtype=MERGE_JOB)
+        self._check(self.lib.amr_dev_consolidate(self.h, j.ctypes.data_as(ctypes.c_void_p),
+                                                 len(j)), "consolidate")
+
+    def reserve_sums(self, rows):
+        self._check(self.lib.amr_dev_reserve_sums(self.h, rows), "reserve_sums")
+
+    def read_sums(self, row0, nrows):
+        out = np.empty((nrows, self.cfg.num_vars), dtype=np.float64)
+        self._check(self.lib.amr_dev_read_sums(self.h, row0, nrows, _ptr(out, ctypes.c_double)),
+                    "read_sums")
+        return out
+
+    def stage(self, start, num, stage=0, permute=False, sum_row=-1):
+        self._check(self.lib.amr_dev_stage(self.h, start, num, stage, int(permute), sum_row),
+                    "stage")
+
+    def sync(self):
+        self._check(self.lib.amr_dev_sync(self.h), "sync")
+
+    def set_timing(self, on=True):
+        self._check(self.lib.amr_dev_set_timing(self.h, int(on)), "set_timing")
+
+    def read_timing(self):
+        ms = np.zeros(4, dtype=np.float64)
+        self._check(self.lib.amr_dev_read_timing(self.h, _ptr(ms, ctypes.c_double)),
+                    "read_timing")
+        return ms
+
+
+def run_driver(args, env=None, cwd=None, timeout=None, exe=EXE_PATH):
+    """Run the drop-in driver with a reference-style argument string."""
+    if not os.path.exists(exe):
+        raise AmrError("%s is not built: run `make -C miniamr_amd`" % exe)
+    e = dict(os.environ)
+    if env:
+        e.update(env)
+    argv = [exe] + (args.split() if isinstance(args, str) else list(args))
+    return subprocess.run(argv, env=e, cwd=cwd, stdout=subprocess.PIPE,
+                          stderr=subprocess.STDOUT, timeout=timeout)

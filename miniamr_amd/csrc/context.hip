@@ -1,0 +1,446 @@
+// context.hip -- context lifetime, host<->device copies, mesh plan upload,
+// checksum table, fused stage entry point and timing for libminiamr_dev.so.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+#include <algorithm>
+
+#include "amr_internal.h"
+
+int amr_set_error(amr_dev *d, int rc, const char *fmt, ...)
+{
+   if (d) {
+      va_list ap;
+      va_start(ap, fmt);
+      vsnprintf(d->err, sizeof(d->err), fmt, ap);
+      va_end(ap);
+   }
+   return rc;
+}
+
+int amr_hip_check(amr_dev *d, hipError_t e, const char *what)
+{
+   return amr_set_error(d, e == hipErrorOutOfMemory ? AMR_ERR_NOMEM : AMR_ERR_HIP,
+                        "%s: %s", what, hipGetErrorString(e));
+}
+
+static hipEvent_t get_event(amr_dev *d)
+{
+   if (!d->ev_free.empty()) {
+      hipEvent_t e = d->ev_free.back();
+      d->ev_free.pop_back();
+      return e;
+   }
+   hipEvent_t e = nullptr;
+   if (hipEventCreate(&e) != hipSuccess)
+      return nullptr;
+   return e;
+}
+
+void amr_span_begin(amr_dev *d, int cat)
+{
+   if (!d->timing)
+      return;
+   amr_dev::Span s;
+   s.a = get_event(d);
+   s.b = nullptr;
+   s.cat = cat;
+   if (s.a)
+      hipEventRecord(s.a, d->stream);
+   d->spans.push_back(s);
+}
+
+void amr_span_end(amr_dev *d)
+{
+   if (!d->timing || d->spans.empty())
+      return;
+   amr_dev::Span &s = d->spans.back();
+   s.b = get_event(d);
+   if (s.b)
+      hipEventRecord(s.b, d->stream);
+}
+
+extern "C" {
+
+int amr_dev_create(const amr_dev_config *cfg, amr_dev **out)
+{
+   if (!out)
+      return AMR_ERR_ARG;
+   *out = nullptr;
+   if (!cfg)
+      return AMR_ERR_ARG;
+   amr_dev *d = new amr_dev();
+   *out = d;
+   d->c = *cfg;
+   if (cfg->nx < 2 || cfg->ny < 2 || cfg->nz < 2 || (cfg->nx | cfg->ny | cfg->nz) & 1)
+      return amr_set_error(d, AMR_ERR_ARG, "block sizes must be even and >= 2");
+   if (cfg->num_vars < 1 || cfg->max_blocks < 1)
+      return amr_set_error(d, AMR_ERR_ARG, "num_vars and max_blocks must be positive");
+   if (cfg->stencil != 7 && cfg->stencil != 27)
+      return amr_set_error(d, AMR_ERR_ARG, "stencil must be 7 or 27");
+   if (cfg->precision < AMR_PREC_F64 || cfg->precision > AMR_PREC_BF16)
+      return amr_set_error(d, AMR_ERR_ARG, "unknown precision %d", cfg->precision);
+
+   int ndev = 0;
+   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+      return amr_set_error(d, AMR_ERR_NODEV, "no HIP device available");
+   if (cfg->device < 0 || cfg->device >= ndev)
+      return amr_set_error(d, AMR_ERR_NODEV, "device %d out of range (%d devices)",
+                           cfg->device, ndev);
+   HIP_TRY(d, hipSetDevice(cfg->device));
+   hipDeviceProp_t prop;
+   HIP_TRY(d, hipGetDeviceProperties(&prop, cfg->device));
+   if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+      return amr_set_error(d, AMR_ERR_NODEV, "device %d is %s, this library is built for gfx950",
+                           cfg->device, prop.gcnArchName);
+
+   Geo &g = d->g;
+   g.nx = cfg->nx; g.ny = cfg->ny; g.nz = cfg->nz;
+   g.gx = g.nx + 2; g.gy = g.ny + 2; g.gz = g.nz + 2;
+   g.stencil = cfg->stencil;
+   d->elem_bytes = cfg->precision == AMR_PREC_F64 ? 8 : cfg->precision == AMR_PREC_F32 ? 4 : 2;
+   int64_t per256 = 256 / d->elem_bytes;
+   int64_t cells = (int64_t) g.gx * g.gy * g.gz;
+   g.slab = (cells + per256 - 1) / per256 * per256;
+   g.var_stride = g.slab * cfg->max_blocks;
+   size_t pool_bytes = (size_t) g.var_stride * cfg->num_vars * d->elem_bytes;
+
+   HIP_TRY(d, hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
+   HIP_TRY(d, hipMalloc(&d->pool, pool_bytes));
+   HIP_TRY(d, hipMemsetAsync(d->pool, 0, pool_bytes, d->stream));
+   HIP_TRY(d, hipMalloc(&d->d_sorted, sizeof(int32_t) * cfg->max_blocks));
+   HIP_TRY(d, hipMalloc(&d->d_partials, sizeof(double) * (size_t) cfg->num_vars * cfg->max_blocks));
+   HIP_TRY(d, hipMalloc(&d->d_stage, sizeof(double) * (size_t) cells * cfg->num_vars));
+   d->h_pinned_bytes = sizeof(double) * std::max<size_t>(4096, (size_t) cfg->num_vars * 64);
+   HIP_TRY(d, hipHostMalloc((void **) &d->h_pinned, d->h_pinned_bytes, hipHostMallocDefault));
+   HIP_TRY(d, hipStreamSynchronize(d->stream));
+   return AMR_OK;
+}
+
+int amr_dev_destroy(amr_dev *d)
+{
+   if (!d)
+      return AMR_OK;
+   if (d->stream)
+      hipStreamSynchronize(d->stream);
+   for (auto &s : d->spans) {
+      if (s.a) hipEventDestroy(s.a);
+      if (s.b) hipEventDestroy(s.b);
+   }
+   for (auto e : d->ev_free)
+      hipEventDestroy(e);
+   hipFree(d->pool);
+   hipFree(d->d_sorted);
+   hipFree(d->d_jobs);
+   hipFree(d->d_partials);
+   hipFree(d->d_sums);
+   hipFree(d->d_stage);
+   hipFree(d->d_scratch);
+   hipFree(d->d_jobbuf);
+   if (d->h_pinned)
+      hipHostFree(d->h_pinned);
+   if (d->stream)
+      hipStreamDestroy(d->stream);
+   delete d;
+   return AMR_OK;
+}
+
+const char *amr_dev_last_error(const amr_dev *d)
+{
+   return d ? d->err : "no context";
+}
+
+int amr_dev_upload_block(amr_dev *d, int slot, const double *cells)
+{
+   if (!d || !cells || slot < 0 || slot >= d->c.max_blocks)
+      return amr_set_error(d, AMR_ERR_ARG, "upload: slot %d out of range", slot);
+   size_t bytes = sizeof(double) * (size_t) d->g.gx * d->g.gy * d->g.gz * d->c.num_vars;
+   HIP_TRY(d, hipMemcpyAsync(d->d_stage, cells, bytes, hipMemcpyHostToDevice, d->stream));
+   int rc = amr_launch_convert_in(d, slot);
+   if (rc)
+      return rc;
+   HIP_TRY(d, hipStreamSynchronize(d->stream));
+   return AMR_OK;
+}
+
+int amr_dev_download_block(amr_dev *d, int slot, double *cells)
+{
+   if (!d || !cells || slot < 0 || slot >= d->c.max_blocks)
+      return amr_set_error(d, AMR_ERR_ARG, "download: slot %d out of range", slot);
+   int rc = amr_launch_convert_out(d, slot);
+   if (rc)
+      return rc;
+   size_t bytes = sizeof(double) * (size_t) d->g.gx * d->g.gy * d->g.gz * d->c.num_vars;
+   HIP_TRY(d, hipMemcpyAsync(cells, d->d_stage, bytes, hipMemcpyDeviceToHost, d->stream));
+   HIP_TRY(d, hipStreamSynchronize(d->stream));
+   return AMR_OK;
+}
+
+int amr_dev_set_mesh(amr_dev *d, const int32_t *sorted, int n_active,
+                     const amr_face_job *jobs, const int32_t count[3])
+{
+   if (!d || n_active < 0 || n_active > d->c.max_blocks || (n_active && !sorted) || !count)
+      return amr_set_error(d, AMR_ERR_ARG, "set_mesh: bad arguments");
+   int nj = 0;
+   for (int i = 0; i < 3; i++) {
+      if (count[i] < 0)
+         return amr_set_error(d, AMR_ERR_ARG, "set_mesh: negative job count");
+      nj += count[i];
+   }
+   if (nj && !jobs)
+      return amr_set_error(d, AMR_ERR_ARG, "set_mesh: jobs missing");
+   // Validate every job on the host: slots in range and faces/quarters legal,
+   // so no kernel can address outside the pool.
+   for (int i = 0; i < n_active; i++)
+      if (sorted[i] < 0 || sorted[i] >= d->c.max_blocks)
+         return amr_set_error(d, AMR_ERR_ARG, "set_mesh: slot %d out of range", sorted[i]);
+   for (int t = 0; t < nj; t++) {
+      const amr_face_job &j = jobs[t];
+      int dir = t < count[0] ? 0 : t < count[0] + count[1] ? 1 : 2;
+      if (j.kind < AMR_FACE_SAME || j.kind > AMR_FACE_BC || j.face / 2 != dir ||
+          j.face < 0 || j.face > 5 || j.dst < 0 || j.dst >= d->c.max_blocks ||
+          j.src < 0 || j.src >= d->c.max_blocks || (j.iq & ~1) || (j.jq & ~1))
+         return amr_set_error(d, AMR_ERR_ARG, "set_mesh: bad face job %d", t);
+   }
+   HIP_TRY(d, hipStreamSynchronize(d->stream));
+   if (n_active)
+      HIP_TRY(d, hipMemcpy(d->d_sorted, sorted, sizeof(int32_t) * n_active,
+                           hipMemcpyHostToDevice));
+   if (nj > d->jobs_cap) {
+      hipFree(d->d_jobs);
+      d->d_jobs = nullptr;
+      int cap = std::max(nj, 2 * d->jobs_cap);
+      HIP_TRY(d, hipMalloc(&d->d_jobs, sizeof(amr_face_job) * cap));
+      d->jobs_cap = cap;
+   }
+   if (nj)
+      HIP_TRY(d, hipMemcpy(d->d_jobs, jobs, sizeof(amr_face_job) * nj, hipMemcpyHostToDevice));
+   d->n_active = n_active;
+   for (int i = 0; i < 3; i++)
+      d->count[i] = count[i];
+   return AMR_OK;
+}
+
+int amr_dev_comm(amr_dev *d, int start, int num, int stage, int permute)
+{
+   static const int perm[6][3] = { {0, 1, 2}, {1, 2, 0}, {2, 0, 1},
+                                   {0, 2, 1}, {1, 0, 2}, {2, 1, 0} };
+   if (!d || start < 0 || num < 0 || start + num > d->c.num_vars || stage < 0)
+      return amr_set_error(d, AMR_ERR_ARG, "comm: variables [%d, %d) out of range", start,
+                           start + num);
+   if (!num)
+      return AMR_OK;
+   amr_span_begin(d, 0);
+   for (int o = 0; o < 3; o++) {
+      int dir = permute ? perm[stage % 6][o] : o;
+      int rc = amr_launch_exchange_dir(d, dir, start, num);
+      if (rc)
+         return rc;
+   }
+   amr_span_end(d);
+   return AMR_OK;
+}
+
+int amr_dev_stencil(amr_dev *d, int var)
+{
+   if (!d || var < 0 || var >= d->c.num_vars)
+      return amr_set_error(d, AMR_ERR_ARG, "stencil: variable %d out of range", var);
+   amr_span_begin(d, 1);
+   int rc = amr_launch_stencil(d, var, 1, false);
+   amr_span_end(d);
+   return rc;
+}
+
+int amr_dev_check_sum(amr_dev *d, int var, double *sum)
+{
+   if (!d || !sum || var < 0 || var >= d->c.num_vars)
+      return amr_set_error(d, AMR_ERR_ARG, "check_sum: variable %d out of range", var);
+   amr_span_begin(d, 2);
+   int rc = amr_launch_block_sums(d, var, 1);
+   if (!rc)
+      rc = amr_launch_reduce_sums(d, var, 1, (double *) d->d_stage);
+   amr_span_end(d);
+   if (rc)
+      return rc;
+   HIP_TRY(d, hipMemcpyAsync(d->h_pinned, d->d_stage, sizeof(double), hipMemcpyDeviceToHost,
+                             d->stream));
+   HIP_TRY(d, hipStreamSynchronize(d->stream));
+   *sum = d->h_pinned[0];
+   return AMR_OK;
+}
+
+}  // extern "C"
+
+// Snapshot scratch for split/merge: every job needs num_vars * (n/2)^3
+// values; batches larger than the cap are processed in order in chunks
+// (safe: a chunk can only overwrite slots freed by itself or earlier
+// chunks, never a slot a later chunk still reads).
+static const size_t kScratchCap = (size_t) 512 << 20;
+
+static int ensure_scratch(amr_dev *d, size_t bytes)
+{
+   if (bytes <= d->scratch_bytes)
+      return AMR_OK;
+   HIP_TRY(d, hipStreamSynchronize(d->stream));
+   hipFree(d->d_scratch);
+   d->d_scratch = nullptr;
+   d->scratch_bytes = 0;
+   HIP_TRY(d, hipMalloc(&d->d_scratch, bytes));
+   d->scratch_bytes = bytes;
+   return AMR_OK;
+}
+
+static int ensure_jobbuf(amr_dev *d, size_t bytes)
+{
+   if (bytes <= d->jobbuf_bytes)
+      return AMR_OK;
+   HIP_TRY(d, hipStreamSynchronize(d->stream));
+   hipFree(d->d_jobbuf);
+   d->d_jobbuf = nullptr;
+   d->jobbuf_bytes = 0;
+   HIP_TRY(d, hipMalloc(&d->d_jobbuf, bytes));
+   d->jobbuf_bytes = bytes;
+   return AMR_OK;
+}
+
+template <typename J>
+static int refine_data(amr_dev *d, const J *jobs, int n, bool split)
+{
+   if (!d || n < 0 || (n && !jobs))
+      return amr_set_error(d, AMR_ERR_ARG, "split/consolidate: bad arguments");
+   if (!n)
+      return AMR_OK;
+   for (int t = 0; t < n; t++) {
+      int a = split ? ((const amr_split_job *) (const void *) &jobs[t])->parent
+                    : ((const amr_merge_job *) (const void *) &jobs[t])->dst;
+      int b = split ? ((const amr_split_job *) (const void *) &jobs[t])->child
+                    : ((const amr_merge_job *) (const void *) &jobs[t])->child;
+      int o = split ? ((const amr_split_job *) (const void *) &jobs[t])->octant
+                    : ((const amr_merge_job *) (const void *) &jobs[t])->octant;
+      if (a < 0 || a >= d->c.max_blocks || b < 0 || b >= d->c.max_blocks || o < 0 || o > 7)
+         return amr_set_error(d, AMR_ERR_ARG, "split/consolidate: bad job %d", t);
+   }
+   size_t per_job = (size_t) d->c.num_vars * (d->g.nx / 2) * (d->g.ny / 2) * (d->g.nz / 2) *
+                    d->elem_bytes;
+   int chunk = (int) std::max<size_t>(1, std::min<size_t>(n, kScratchCap / per_job));
+   int rc = ensure_scratch(d, per_job * chunk);
+   if (!rc)
+      rc = ensure_jobbuf(d, sizeof(J) * n);
+   if (rc)
+      return rc;
+   HIP_TRY(d, hipStreamSynchronize(d->stream));
+   HIP_TRY(d, hipMemcpy(d->d_jobbuf, jobs, sizeof(J) * n, hipMemcpyHostToDevice));
+   amr_span_begin(d, 3);
+   for (int t0 = 0; t0 < n && !rc; t0 += chunk) {
+      int m = std::min(chunk, n - t0);
+      const J *dj = (const J *) d->d_jobbuf + t0;
+      rc = split ? amr_launch_split(d, (const amr_split_job *) dj, m, d->d_scratch)
+                 : amr_launch_merge(d, (const amr_merge_job *) dj, m, d->d_scratch);
+   }
+   amr_span_end(d);
+   return rc;
+}
+
+extern "C" {
+
+int amr_dev_split(amr_dev *d, const amr_split_job *jobs, int n)
+{
+   return refine_data(d, jobs, n, true);
+}
+
+int amr_dev_consolidate(amr_dev *d, const amr_merge_job *jobs, int n)
+{
+   return refine_data(d, jobs, n, false);
+}
+
+int amr_dev_reserve_sums(amr_dev *d, int n_rows)
+{
+   if (!d || n_rows < 1)
+      return amr_set_error(d, AMR_ERR_ARG, "reserve_sums: n_rows must be positive");
+   if (n_rows <= d->sum_rows)
+      return AMR_OK;
+   HIP_TRY(d, hipStreamSynchronize(d->stream));
+   hipFree(d->d_sums);
+   d->d_sums = nullptr;
+   d->sum_rows = 0;
+   HIP_TRY(d, hipMalloc(&d->d_sums, sizeof(double) * (size_t) n_rows * d->c.num_vars));
+   HIP_TRY(d, hipMemset(d->d_sums, 0, sizeof(double) * (size_t) n_rows * d->c.num_vars));
+   d->sum_rows = n_rows;
+   size_t need = sizeof(double) * (size_t) n_rows * d->c.num_vars;
+   if (need > d->h_pinned_bytes) {
+      hipHostFree(d->h_pinned);
+      d->h_pinned = nullptr;
+      HIP_TRY(d, hipHostMalloc((void **) &d->h_pinned, need, hipHostMallocDefault));
+      d->h_pinned_bytes = need;
+   }
+   return AMR_OK;
+}
+
+int amr_dev_read_sums(amr_dev *d, int row0, int n_rows, double *out)
+{
+   if (!d || !out || row0 < 0 || n_rows < 0 || row0 + n_rows > d->sum_rows)
+      return amr_set_error(d, AMR_ERR_ARG, "read_sums: rows [%d, %d) out of range", row0,
+                           row0 + n_rows);
+   size_t bytes = sizeof(double) * (size_t) n_rows * d->c.num_vars;
+   HIP_TRY(d, hipMemcpyAsync(d->h_pinned, d->d_sums + (size_t) row0 * d->c.num_vars, bytes,
+                             hipMemcpyDeviceToHost, d->stream));
+   HIP_TRY(d, hipStreamSynchronize(d->stream));
+   memcpy(out, d->h_pinned, bytes);
+   return AMR_OK;
+}
+
+int amr_dev_stage(amr_dev *d, int start, int num, int stage, int permute, int sum_row)
+{
+   if (!d || start < 0 || num < 0 || start + num > d->c.num_vars)
+      return amr_set_error(d, AMR_ERR_ARG, "stage: variables out of range");
+   if (sum_row >= d->sum_rows)
+      return amr_set_error(d, AMR_ERR_ARG, "stage: checksum row %d not reserved", sum_row);
+   int rc = amr_dev_comm(d, start, num, stage, permute);
+   if (rc)
+      return rc;
+   amr_span_begin(d, 1);
+   rc = amr_launch_stencil(d, start, num, sum_row >= 0);
+   amr_span_end(d);
+   if (rc || sum_row < 0)
+      return rc;
+   amr_span_begin(d, 2);
+   rc = amr_launch_reduce_sums(d, start, num,
+                               d->d_sums + (size_t) sum_row * d->c.num_vars + start);
+   amr_span_end(d);
+   return rc;
+}
+
+int amr_dev_sync(amr_dev *d)
+{
+   if (!d)
+      return AMR_ERR_ARG;
+   HIP_TRY(d, hipStreamSynchronize(d->stream));
+   return AMR_OK;
+}
+
+int amr_dev_set_timing(amr_dev *d, int on)
+{
+   if (!d)
+      return AMR_ERR_ARG;
+   d->timing = on != 0;
+   return AMR_OK;
+}
+
+int amr_dev_read_timing(amr_dev *d, double ms[4])
+{
+   if (!d || !ms)
+      return AMR_ERR_ARG;
+   ms[0] = ms[1] = ms[2] = ms[3] = 0.0;
+   HIP_TRY(d, hipStreamSynchronize(d->stream));
+   for (auto &s : d->spans) {
+      float t = 0.0f;
+      if (s.a && s.b && hipEventElapsedTime(&t, s.a, s.b) == hipSuccess)
+         ms[s.cat] += t;
+      if (s.a) d->ev_free.push_back(s.a);
+      if (s.b) d->ev_free.push_back(s.b);
+   }
+   d->spans.clear();
+   return AMR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,141 @@
+// refine_data.hip -- cell data of block split and consolidate.
+//
+// split (ref/block.c:161-173): every child cell of octant o is the parent
+// cell it lies in divided by 8.  consolidate (ref/block.c:411-431): every
+// cell of the merged block's octant o is the sum of the 2x2x2 child cells
+// it covers, added in the reference's order (x fastest, then y, then z).
+// Only interior cells are written; a new block's ghost cells keep what the
+// slot held before, as in the reference.
+//
+// A batch is applied "as if one job after another": a slot freed by an
+// earlier job (a split parent, a consolidated child) can be the target of
+// a later job.  Each launch pair therefore first snapshots what the batch
+// reads (the parent octants, or the merged sums) into scratch and only then
+// writes the targets, so reads never see the batch's own writes.
+#include "amr_internal.h"
+
+namespace {
+
+// Snapshot of parent octants: scratch[job][var][hx][hy][hz].
+template <typename T>
+__global__ void k_split_snap(const T *__restrict__ pool, const amr_split_job *__restrict__ jobs,
+                             T *__restrict__ scratch, Geo g, int nvars)
+{
+   const amr_split_job j = jobs[blockIdx.x];
+   const int var = blockIdx.y;
+   const int hx = g.nx / 2, hy = g.ny / 2, hz = g.nz / 2, hc = hx * hy * hz;
+   const int i1 = (j.octant & 1) * hx, j1 = ((j.octant >> 1) & 1) * hy, k1 = (j.octant >> 2) * hz;
+   const T *p = pool + (int64_t) var * g.var_stride + (int64_t) j.parent * g.slab;
+   T *s = scratch + ((int64_t) blockIdx.x * nvars + var) * hc;
+   for (int t = threadIdx.x; t < hc; t += blockDim.x) {
+      const int i = t / (hy * hz), r = t - i * hy * hz, jj = r / hz, k = r - jj * hz;
+      s[t] = p[((int64_t) (i + 1 + i1) * g.gy + (jj + 1 + j1)) * g.gz + (k + 1 + k1)];
+   }
+}
+
+template <typename T>
+__global__ void k_split_write(T *__restrict__ pool, const amr_split_job *__restrict__ jobs,
+                              const T *__restrict__ scratch, Geo g, int nvars)
+{
+   typedef typename Acc<T>::type A;
+   const amr_split_job j = jobs[blockIdx.x];
+   const int var = blockIdx.y;
+   const int hy = g.ny / 2, hz = g.nz / 2, hc = (g.nx / 2) * hy * hz;
+   T *c = pool + (int64_t) var * g.var_stride + (int64_t) j.child * g.slab;
+   const T *s = scratch + ((int64_t) blockIdx.x * nvars + var) * hc;
+   const int nyz = g.ny * g.nz, ncell = g.nx * nyz;
+   for (int t = threadIdx.x; t < ncell; t += blockDim.x) {
+      const int i = t / nyz, r = t - i * nyz, jj = r / g.nz, k = r - jj * g.nz;
+      const A v = to_acc(s[((i >> 1) * hy + (jj >> 1)) * hz + (k >> 1)]) / A(8);
+      c[((int64_t) (i + 1) * g.gy + (jj + 1)) * g.gz + (k + 1)] = from_acc<T>(v);
+   }
+}
+
+// Merged sums: scratch[job][var][hx][hy][hz].
+template <typename T>
+__global__ void k_merge_sum(const T *__restrict__ pool, const amr_merge_job *__restrict__ jobs,
+                            T *__restrict__ scratch, Geo g, int nvars)
+{
+   typedef typename Acc<T>::type A;
+   const amr_merge_job j = jobs[blockIdx.x];
+   const int var = blockIdx.y;
+   const int hy = g.ny / 2, hz = g.nz / 2, hc = (g.nx / 2) * hy * hz;
+   const T *c = pool + (int64_t) var * g.var_stride + (int64_t) j.child * g.slab;
+   T *s = scratch + ((int64_t) blockIdx.x * nvars + var) * hc;
+   const int64_t sx = (int64_t) g.gy * g.gz, sy = g.gz;
+   for (int t = threadIdx.x; t < hc; t += blockDim.x) {
+      const int i = t / (hy * hz), r = t - i * hy * hz, jj = r / hz, k = r - jj * hz;
+      const int64_t o = (2 * i + 1) * sx + (2 * jj + 1) * sy + (2 * k + 1);
+      A v = to_acc(c[o]);
+      v = v + to_acc(c[o + sx]);
+      v = v + to_acc(c[o + sy]);
+      v = v + to_acc(c[o + sx + sy]);
+      v = v + to_acc(c[o + 1]);
+      v = v + to_acc(c[o + sx + 1]);
+      v = v + to_acc(c[o + sy + 1]);
+      v = v + to_acc(c[o + sx + sy + 1]);
+      s[t] = from_acc<T>(v);
+   }
+}
+
+template <typename T>
+__global__ void k_merge_write(T *__restrict__ pool, const amr_merge_job *__restrict__ jobs,
+                              const T *__restrict__ scratch, Geo g, int nvars)
+{
+   const amr_merge_job j = jobs[blockIdx.x];
+   const int var = blockIdx.y;
+   const int hx = g.nx / 2, hy = g.ny / 2, hz = g.nz / 2, hc = hx * hy * hz;
+   const int i1 = (j.octant & 1) * hx, j1 = ((j.octant >> 1) & 1) * hy, k1 = (j.octant >> 2) * hz;
+   T *p = pool + (int64_t) var * g.var_stride + (int64_t) j.dst * g.slab;
+   const T *s = scratch + ((int64_t) blockIdx.x * nvars + var) * hc;
+   for (int t = threadIdx.x; t < hc; t += blockDim.x) {
+      const int i = t / (hy * hz), r = t - i * hy * hz, jj = r / hz, k = r - jj * hz;
+      p[((int64_t) (i + 1 + i1) * g.gy + (jj + 1 + j1)) * g.gz + (k + 1 + k1)] = s[t];
+   }
+}
+
+template <typename T>
+int split(amr_dev *d, const amr_split_job *jobs, int n, void *scratch)
+{
+   dim3 grid(n, d->c.num_vars);
+   hipLaunchKernelGGL(k_split_snap<T>, grid, dim3(256), 0, d->stream, (const T *) d->pool, jobs,
+                      (T *) scratch, d->g, d->c.num_vars);
+   HIP_TRY(d, hipGetLastError());
+   hipLaunchKernelGGL(k_split_write<T>, grid, dim3(256), 0, d->stream, (T *) d->pool, jobs,
+                      (const T *) scratch, d->g, d->c.num_vars);
+   HIP_TRY(d, hipGetLastError());
+   return AMR_OK;
+}
+
+template <typename T>
+int merge(amr_dev *d, const amr_merge_job *jobs, int n, void *scratch)
+{
+   dim3 grid(n, d->c.num_vars);
+   hipLaunchKernelGGL(k_merge_sum<T>, grid, dim3(256), 0, d->stream, (const T *) d->pool, jobs,
+                      (T *) scratch, d->g, d->c.num_vars);
+   HIP_TRY(d, hipGetLastError());
+   hipLaunchKernelGGL(k_merge_write<T>, grid, dim3(256), 0, d->stream, (T *) d->pool, jobs,
+                      (const T *) scratch, d->g, d->c.num_vars);
+   HIP_TRY(d, hipGetLastError());
+   return AMR_OK;
+}
+
+}  // namespace
+
+int amr_launch_split(amr_dev *d, const amr_split_job *jobs, int n, void *scratch)
+{
+   switch (d->c.precision) {
+   case AMR_PREC_F64: return split<double>(d, jobs, n, scratch);
+   case AMR_PREC_F32: return split<float>(d, jobs, n, scratch);
+   default: return split<uint16_t>(d, jobs, n, scratch);
+   }
+}
+
+int amr_launch_merge(amr_dev *d, const amr_merge_job *jobs, int n, void *scratch)
+{
+   switch (d->c.precision) {
+   case AMR_PREC_F64: return merge<double>(d, jobs, n, scratch);
+   case AMR_PREC_F32: return merge<float>(d, jobs, n, scratch);
+   default: return merge<uint16_t>(d, jobs, n, scratch);
+   }
+}

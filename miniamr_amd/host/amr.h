@@ -16,6 +16,7 @@
 
 #include <stdint.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include "amr_dev.h"
 
 #define AMR_BOUNDARY (-2)   /* nei_level of a face on the domain boundary */
@@ -103,6 +104,14 @@ typedef struct {
    /* timers (seconds) in the reference's report slots */
    double t_all, t_init, t_refine_all, t_comm, t_calc, t_cs, t_main;
 
+   /* The reference draws its initial values from libc rand() with the
+    * implicit seed 1 (ref/init.c:494).  The GPU runtime calls srand()/rand()
+    * itself while it initialises, so the driver keeps a private state of the
+    * same glibc generator, seeded as rand() is by default:
+    * initstate_r(1, state, 128) yields rand()'s sequence exactly. */
+   struct random_data rng;
+   char rng_state[128];
+
    /* device */
    amr_dev *dev;
    amr_vec split_jobs, merge_jobs;
@@ -124,6 +133,7 @@ int amr_init(amr_run *R);
 void amr_sorted_add(amr_run *R, int slot, int64_t number, int level);
 void amr_sorted_del(amr_run *R, int64_t number, int level);
 int amr_n_active(const amr_run *R);
+double amr_rand01(amr_run *R);
 int amr_build_exchange_plan(amr_run *R);
 int amr_sync_mesh(amr_run *R);
 void amr_fatal(const char *fmt, ...);

@@ -3,9 +3,9 @@
  *
  * Initial mesh: ref/init.c:38-683 for one rank with the RCB/Morton layout
  * (lb_method < 2): blocks numbered x-fastest, slot == number, level 0,
- * reflective faces on the domain boundary, interior values drawn from libc
- * rand() in the reference's order (block, var, i, j, k) so that the same
- * binary on the same libc produces the same field.
+ * reflective faces on the domain boundary, interior values drawn from the
+ * glibc generator behind rand() with its default seed, in the reference's
+ * order (block, var, i, j, k), so the field equals the reference's.
  *
  * Sorted list: ref/block.c:610-657 -- active blocks grouped by level, each
  * level in descending block number.  Its order fixes the checksum
@@ -170,6 +170,8 @@ int amr_init(amr_run *R)
    int nr = P->num_refine;
 
    R->tol = pow(10.0, (double) -P->error_tol);
+   memset(&R->rng, 0, sizeof(R->rng));
+   initstate_r(1, R->rng_state, sizeof(R->rng_state), &R->rng);
    R->p2[0] = 1;
    R->p8[0] = 1;
    for (int i = 0; i <= nr; i++) {
@@ -220,15 +222,14 @@ int amr_init(amr_run *R)
             b->cen[1] = j * size + size / 2;
             b->cen[2] = k * size + size / 2;
             amr_sorted_add(R, n, n, 0);
-            /* zero the whole slot, then the interior from rand() */
+            /* zero the whole slot, then the interior from the generator */
             memset(R->cellbuf, 0, sizeof(double) * slab * P->num_vars);
             for (int v = 0; v < P->num_vars; v++) {
                double *a = R->cellbuf + slab * v;
                for (int x = 1; x <= P->nx; x++)
                   for (int y = 1; y <= P->ny; y++)
                      for (int z = 1; z <= P->nz; z++)
-                        a[((size_t) x * gy + y) * gz + z] =
-                           ((double) rand()) / ((double) RAND_MAX);
+                        a[((size_t) x * gy + y) * gz + z] = amr_rand01(R);
             }
             amr_dev_check(R, amr_dev_upload_block(R->dev, n, R->cellbuf), "upload");
             memset(b->nei, 0, sizeof(b->nei));
@@ -255,6 +256,14 @@ int amr_init(amr_run *R)
    for (int v = 0; v < P->num_vars; v++)
       R->grid_sum[v] = amr_checksum(R, v);
    return 0;
+}
+
+/* rand()/RAND_MAX from the private generator (see amr_run.rng). */
+double amr_rand01(amr_run *R)
+{
+   int32_t r;
+   random_r(&R->rng, &r);
+   return ((double) r) / ((double) RAND_MAX);
 }
 
 static int opposite(int f) { return f ^ 1; }

@@ -1,0 +1,182 @@
+"""Parity of the HIP path (GPU).
+
+Kernel level: the product library and the CPU oracle receive the same
+inputs through the same C ABI (include/amr_dev.h) and must agree bit for
+bit on every cell for the FP64 path (checksums: 1e-13 relative, since the
+device reduces in a different order).
+
+Driver level: bin/miniAMR.x (host driver + HIP library) against every
+golden case of the reference -- checksums within 1e-12 relative, cells
+within 4 ulp (observed: 0), block counts and slots identical.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import amrcheck as A
+import meshgen
+from conftest import require
+import miniamr_amd as M
+
+pytestmark = pytest.mark.gpu
+
+CASES = sorted(f[:-5] for f in os.listdir(A.GOLDEN) if f.endswith(".json"))
+
+
+@pytest.fixture(scope="module")
+def libs():
+    require(M.LIB_PATH, "run __graft_entry__.build()")
+    require(A.ORACLE_SO, "run __graft_entry__.build()")
+    return M.load_library(M.LIB_PATH), M.load_library(A.ORACLE_SO)
+
+
+def pair(libs, **kw):
+    return (M.Device(lib=libs[0], **kw), M.Device(lib=libs[1], **kw))
+
+
+def assert_same_cells(g, c, slots):
+    for s in slots:
+        a, b = g.download(s), c.download(s)
+        u = A.ulp_diff(a, b)
+        assert u.max() == 0, "slot %d differs by %d ulp at %s" % (
+            s, u.max(), np.unravel_index(int(u.argmax()), u.shape))
+
+
+@pytest.mark.parametrize("st", [7, 27])
+@pytest.mark.parametrize("permute", [False, True])
+def test_exchange_matches_oracle(libs, st, permute):
+    print("case: exchange stencil=%d permute=%s" % (st, permute), flush=True)
+    rng = np.random.default_rng(10 + st + permute)
+    n_src, n_dst = 5, 6
+    kw = dict(nx=6, ny=4, nz=8, num_vars=3, max_blocks=n_src + n_dst + 2, stencil=st)
+    g, c = pair(libs, **kw)
+    for s in range(n_src + n_dst):
+        cells = rng.random(g.shape)
+        g.upload(s, cells)
+        c.upload(s, cells)
+    plan = meshgen.random_plan(rng, n_src, n_dst)
+    order = list(range(n_src + n_dst))
+    for d in (g, c):
+        d.set_mesh(order, plan)
+    for stage in range(3):
+        for d in (g, c):
+            d.comm(0, 2, stage=stage, permute=permute)   # variables 0-1 only
+    assert_same_cells(g, c, range(n_src + n_dst))
+
+
+@pytest.mark.parametrize("st", [7, 27])
+@pytest.mark.parametrize("n", [(4, 4, 4), (10, 10, 10), (16, 16, 16), (6, 2, 8)])
+def test_stencil_and_checksum_match_oracle(libs, st, n):
+    print("case: stencil=%d block=%s" % (st, n), flush=True)
+    rng = np.random.default_rng(20 + st)
+    nb = 5
+    kw = dict(nx=n[0], ny=n[1], nz=n[2], num_vars=2, max_blocks=nb + 3, stencil=st)
+    g, c = pair(libs, **kw)
+    for s in range(nb):
+        cells = rng.random(g.shape)
+        g.upload(s + 2, cells)
+        c.upload(s + 2, cells)
+    order = [6, 2, 5, 3, 4]
+    for d in (g, c):
+        d.set_mesh(order, [np.zeros(0, M.FACE_JOB)] * 3)
+        d.stencil(1)
+        d.stencil(1)
+    assert_same_cells(g, c, order)
+    for v in range(2):
+        a, b = g.check_sum(v), c.check_sum(v)
+        assert abs(a - b) <= 1e-13 * abs(b)
+
+
+def test_stage_with_checksum_table(libs):
+    rng = np.random.default_rng(30)
+    n_src, n_dst = 4, 4
+    kw = dict(nx=4, ny=4, nz=4, num_vars=4, max_blocks=n_src + n_dst, stencil=27)
+    g, c = pair(libs, **kw)
+    for s in range(n_src + n_dst):
+        cells = rng.random(g.shape)
+        g.upload(s, cells)
+        c.upload(s, cells)
+    plan = meshgen.random_plan(rng, n_src, n_dst)
+    for d in (g, c):
+        d.set_mesh(list(range(n_src, n_src + n_dst)), plan)
+        d.reserve_sums(3)
+        d.stage(0, 4, stage=0, sum_row=0)
+        d.stage(1, 2, stage=1, sum_row=-1)
+        d.stage(0, 3, stage=2, permute=True, sum_row=2)
+    assert_same_cells(g, c, range(n_src + n_dst))
+    sg, sc = g.read_sums(0, 3), c.read_sums(0, 3)
+    np.testing.assert_allclose(sg[[0]], sc[[0]], rtol=1e-13)
+    np.testing.assert_allclose(sg[2, :3], sc[2, :3], rtol=1e-13)
+
+
+def test_split_and_consolidate_match_oracle(libs):
+    rng = np.random.default_rng(40)
+    nparents = 5
+    mb = meshgen.split_slots_needed(nparents) + 8 * 3 + 1
+    kw = dict(nx=4, ny=6, nz=8, num_vars=3, max_blocks=mb, stencil=7)
+    g, c = pair(libs, **kw)
+    for s in range(mb):
+        cells = rng.random(g.shape)
+        g.upload(s, cells)
+        c.upload(s, cells)
+    jobs = meshgen.split_jobs_with_reuse(rng, nparents)
+    for d in (g, c):
+        d.split(jobs)
+    assert_same_cells(g, c, range(mb))
+    merge = meshgen.merge_jobs_with_reuse(rng, 3)
+    for d in (g, c):
+        d.consolidate(merge)
+    assert_same_cells(g, c, range(mb))
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_driver_matches_reference(name):
+    require(M.EXE_PATH, "run __graft_entry__.build()")
+    g = A.golden(name)
+    print("case: driver golden %s: %s" % (name, g["args"]), flush=True)
+    kind = "cells" if "cells" in g else ("bsums" if "bsums" in g else None)
+    r = A.run(M.EXE_PATH, g["args"], dump=kind, timeout=110)
+    check_cells = not any("difference too large" in m for m in g["messages"])
+    rep = A.compare(r, g, sum_rtol=1e-12, max_ulp=4, check_cells=check_cells)
+    print(name, rep)
+
+
+@pytest.mark.parametrize("name", ["t27_perm", "t27_cv", "t7_r3", "cfg4"])
+def test_variable_major_schedule_is_identical(name):
+    require(M.EXE_PATH, "run __graft_entry__.build()")
+    g = A.golden(name)
+    print("case: schedule %s" % name, flush=True)
+    a = A.run(M.EXE_PATH, g["args"], dump="cells", timeout=110)
+    b = A.run(M.EXE_PATH, g["args"], dump="cells", timeout=110,
+              env={"MINIAMR_SCHEDULE": "var", "MINIAMR_VAR_GROUP": "1"})
+    assert a["sums"] == b["sums"]
+    assert np.array_equal(a["cells"]["cells"], b["cells"]["cells"])
+
+
+@pytest.mark.parametrize("prec,tol", [("f32", 1e-6), ("bf16", None)])
+def test_precision_form_checksums(prec, tol):
+    """FP32/BF16 storage with FP32 sums and FP64 checksums against the FP64
+    reference (cfg0: 7-point, 40 variables, 400 stages)."""
+    require(M.EXE_PATH, "run __graft_entry__.build()")
+    g = A.golden("cfg0")
+    print("case: precision %s on cfg0" % prec, flush=True)
+    # BF16's 8-bit mantissa moves the checksum by far more than the default
+    # conservation tolerance (1e-8), so its run relaxes the check.
+    args = g["args"] + ("" if prec == "f32" else " --error_tol 1")
+    r = A.run(M.EXE_PATH, args, dump="bsums", timeout=110, env={"MINIAMR_PRECISION": prec})
+    assert r["rc"] == 0, r["stdout"][-2000:]
+    assert r["counts"] == g["counts"]
+    assert len(r["sums"]) == len(g["sums"]), r["stdout"][-2000:]
+    e = A.rel_err(r["sums"], g["sums"])
+    b = g["bsums"]
+    cells = r["cells"]["cells"]
+    vals = cells[b["sample_block"], b["sample_var"], b["sample_idx"][:, 0],
+                 b["sample_idx"][:, 1], b["sample_idx"][:, 2]]
+    cell_err = np.abs(vals - b["sample_val"]).max()
+    print("precision %s: max checksum rel err vs FP64 reference %.3e, max cell abs err "
+          "(4096 sampled cells) %.3e" % (prec, e.max(), cell_err))
+    if tol is not None:
+        assert e.max() <= tol
+    else:
+        assert e.max() < 5e-2
