@@ -1,0 +1,204 @@
+#!/usr/bin/env python3
+"""Benchmark: miniAMR 27-point, 4x4x4 initial blocks of 16^3 cells, 40
+variables, 4 refinement levels, two moving objects (the bandwidth-scale
+workload, ~3-5.5k active blocks) on one MI355X.
+
+A *step* is one timestep of the drop-in driver: 20 stages, each an
+exchange of all 40 variables plus the 27-point stencil over every active
+block (checksums every 5 stages), then object motion and, every 4th
+timestep, refinement.  The workload has 10 timesteps, so the default
+--warmup 2 --steps 8 times timesteps 3..10 of the real run.  Cells start
+resident in HBM; nothing crosses PCIe inside the timed region except the
+per-timestep checksum read-back (40 doubles per checksum stage).
+
+value = cell-variable updates per second over the timed timesteps
+        (updates = sum over stages of active blocks x 16^3 x 40).
+roofline: the stencil kernel.  Algorithmic bytes per launch = one read and
+        one write of every updated cell (2 x 8 B in FP64, SURVEY B_alg);
+        achieved = those bytes / the stencil's device time measured with
+        HIP events on the library's stream over the timed region.
+cpu_baseline: the reference's OpenMP build (oracle/_ref/miniAMR_omp.x) on
+        this host, one MPI rank, on a bounded sample of the same workload
+        (its first timestep, --num_tsteps 1), rate from its Summary line.
+"""
+import argparse
+import ctypes
+import json
+import os
+import re
+import subprocess
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+import miniamr_amd as M  # noqa: E402
+
+CFG1 = ("--init_x 4 --init_y 4 --init_z 4 --nx 16 --ny 16 --nz 16 --num_vars 40 "
+        "--stencil 27 --num_tsteps 10 --stages_per_ts 20 --checksum_freq 5 --num_refine 4 "
+        "--refine_freq 4 --max_blocks 8000 --error_tol -10 --num_objects 2 "
+        "--object 2 0 0.15 0.15 0.15 0.02 0.02 0.02 0.1 0.1 0.1 0 0 0 "
+        "--object 0 0 0.5 0.5 0.5 0.02 0.02 0.02 0.05 0.05 0.05 0 0 0")
+HBM_PEAK_GBS = 8000.0
+PREC_NAME = {0: "f64", 1: "f32", 2: "bf16"}
+PREC_BYTES = {0: 8, 1: 4, 2: 2}
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("timesteps_done", ctypes.c_int64), ("stages_done", ctypes.c_int64),
+                ("block_stages", ctypes.c_int64), ("cell_var_updates", ctypes.c_double),
+                ("active_blocks", ctypes.c_int64), ("max_slots", ctypes.c_int64),
+                ("device_ms", ctypes.c_double * 4), ("nx", ctypes.c_int), ("ny", ctypes.c_int),
+                ("nz", ctypes.c_int), ("num_vars", ctypes.c_int), ("stencil", ctypes.c_int),
+                ("precision", ctypes.c_int)]
+
+
+def load_host():
+    path = os.path.join(M.PKG, "lib", "libminiamr_host.so")
+    if not os.path.exists(path):
+        raise M.AmrError("%s is not built: run `make -C miniamr_amd`" % path)
+    lib = ctypes.CDLL(path)
+    lib.amr_run_create.restype = ctypes.c_void_p
+    lib.amr_run_create.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p)]
+    for name in ("amr_run_step", "amr_run_finish"):
+        getattr(lib, name).restype = ctypes.c_int
+        getattr(lib, name).argtypes = [ctypes.c_void_p]
+    lib.amr_run_stats_get.restype = ctypes.c_int
+    lib.amr_run_stats_get.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stats)]
+    lib.amr_run_report.argtypes = [ctypes.c_void_p]
+    lib.amr_run_destroy.argtypes = [ctypes.c_void_p]
+    return lib
+
+
+def stats(lib, run):
+    s = Stats()
+    lib.amr_run_stats_get(run, ctypes.byref(s))
+    return s
+
+
+def cpu_baseline(timeout=240):
+    """The reference OpenMP build on a bounded sample: the first timestep."""
+    exe = os.path.join(REPO, "oracle", "_ref", "miniAMR_omp.x")
+    if not os.path.exists(exe):
+        return None
+    threads = min(16, os.cpu_count() or 1)
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close")
+    args = CFG1.replace("--num_tsteps 10", "--num_tsteps 1").split()
+    try:
+        p = subprocess.run([exe] + args, env=env, stdout=subprocess.PIPE,
+                           stderr=subprocess.STDOUT, timeout=timeout, cwd="/tmp")
+    except (subprocess.TimeoutExpired, OSError) as e:
+        print("cpu baseline skipped: %s" % e, file=sys.stderr)
+        return None
+    out = p.stdout.decode(errors="replace")
+    m = re.search(r"Summary: ranks \d+(?: threads \d+)? ts (\d+) time (\S+) calc (\S+) "
+                  r"comm \S+ red \S+ refine \S+ blocks/ts (\S+)", out)
+    if not m:
+        print("cpu baseline: no Summary line (rc %d)" % p.returncode, file=sys.stderr)
+        return None
+    ts, t, calc, bpts = int(m.group(1)), float(m.group(2)), float(m.group(3)), float(m.group(4))
+    updates = bpts * ts * 20 * 16 ** 3 * 40
+    return {"value": updates / t / 1e9, "unit": "GCell-var updates/s", "cores": threads,
+            "kind": "reference",
+            "sample": "reference openmp/ build, 1 MPI rank, %d OpenMP threads: timestep 1 of "
+                      "the same workload (--num_tsteps 1; %.0f blocks/stage, %.3e updates, "
+                      "%.2f s total, %.2f s stencil-only => %.3f GCell-var/s stencil-only)"
+                      % (threads, bpts, updates, t, calc, updates / calc / 1e9)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--precision", default=os.environ.get("MINIAMR_PRECISION", "f64"),
+                    choices=["f64", "f32", "bf16"])
+    ap.add_argument("--schedule", default=os.environ.get("MINIAMR_SCHEDULE", "stage"),
+                    choices=["stage", "var"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py: this build runs one rank on one GPU; --gpus %d is not supported "
+              "(the multi-GPU decomposition is a design, see DESIGN.md)" % a.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    if a.warmup + a.steps > 10:
+        print("bench.py: the workload has 10 timesteps; warmup + steps must be <= 10",
+              file=sys.stderr)
+        sys.exit(2)
+    os.environ["MINIAMR_PRECISION"] = a.precision
+    os.environ["MINIAMR_SCHEDULE"] = a.schedule
+
+    lib = load_host()
+    argv = [b"miniAMR.x"] + [x.encode() for x in CFG1.split()] + [b"--report_perf", b"0"]
+    arr = (ctypes.c_char_p * len(argv))(*argv)
+    t_setup = time.time()
+    run = lib.amr_run_create(len(argv), arr)
+    for _ in range(a.warmup):
+        lib.amr_run_step(run)
+    s0 = stats(lib, run)
+    setup_s = time.time() - t_setup
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        rc = lib.amr_run_step(run)
+        if rc == 2:
+            raise SystemExit("conservation check stopped the run")
+    t1 = time.perf_counter()   # amr_run_step returns after the device is idle
+    s1 = stats(lib, run)
+    lib.amr_run_finish(run)
+    lib.amr_run_destroy(run)
+
+    elapsed = t1 - t0
+    updates = s1.cell_var_updates - s0.cell_var_updates
+    value = updates / elapsed / 1e9
+    ebytes = PREC_BYTES[s1.precision]
+    stencil_s = (s1.device_ms[1] - s0.device_ms[1]) * 1e-3
+    launches = (s1.stages_done - s0.stages_done) * (1 if a.schedule == "stage" else 40)
+    achieved = updates * 2 * ebytes / stencil_s / 1e9 if stencil_s > 0 else None
+    traffic = None
+    pmc = os.path.join(REPO, "profiles", "stencil27_cfg1_%s_pmc.json" % a.precision)
+    if os.path.exists(pmc):
+        with open(pmc) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    line = {
+        "metric": "GCell-var updates/s",
+        "value": round(value, 4),
+        "unit": "GCell-var updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(elapsed / a.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": PREC_NAME[s1.precision],
+        "data": "synthetic: uniform [0,1) cell values from the reference's generator and seed",
+        "config": {"workload": "miniAMR 27-pt, init 4x4x4 blocks of 16^3, 40 vars, "
+                               "num_refine 4, 2 moving objects, timesteps %d-%d of 10"
+                               % (a.warmup + 1, a.warmup + a.steps),
+                   "schedule": a.schedule,
+                   "avg_active_blocks": round((s1.block_stages - s0.block_stages) /
+                                              max(1, s1.stages_done - s0.stages_done), 1),
+                   "cell_var_updates": updates},
+        "roofline": {"bound": "hbm",
+                     "achieved": round(achieved, 1) if achieved else None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+                     "traffic": traffic,
+                     "kernel": "k_stencil (27-point), %d launches, %.3f ms avg" % (
+                         launches, stencil_s / max(1, launches) * 1e3),
+                     "step_level_frac": round(updates * 2 * ebytes / elapsed / 1e9 /
+                                              HBM_PEAK_GBS, 4)},
+        "device_ms": {"exchange": round(s1.device_ms[0] - s0.device_ms[0], 2),
+                      "stencil": round(s1.device_ms[1] - s0.device_ms[1], 2),
+                      "checksum": round(s1.device_ms[2] - s0.device_ms[2], 2),
+                      "split_merge": round(s1.device_ms[3] - s0.device_ms[3], 2)},
+        "setup_s": round(setup_s, 2),
+        "cpu_baseline": None if a.no_cpu_baseline else cpu_baseline(),
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,59 @@
+/*
+ * amr_run.h -- driver-level C API of the miniAMR drop-in (one rank, one GPU).
+ *
+ * The reference's program is main() -> driver() (ref/main.c:38-362,
+ * ref/driver.c:37-165).  This API exposes the same run, step by step, to
+ * callers that want to time or drive it (the benchmark, Python via ctypes):
+ *
+ *   amr_run_create   parse the reference's command line, validate it
+ *                    (ref/main.c:657-723), allocate, build the initial mesh
+ *                    and values and run the initial refinement
+ *                    (ref/driver.c:45-56);
+ *   amr_run_step     one timestep: all stages (exchange, stencil, checksum
+ *                    and conservation test), then object motion and, every
+ *                    refine_freq timesteps, refinement (ref/driver.c:67-160);
+ *   amr_run_finish   end-of-run bookkeeping; amr_run_report prints the
+ *                    Summary line (ref/profile.c:1298);
+ *   amr_run_destroy  free everything.
+ *
+ * Invalid command lines terminate the process with the reference's message
+ * and exit status, as the reference does.
+ */
+#ifndef AMR_RUN_H
+#define AMR_RUN_H
+
+#include <stdint.h>
+#include "amr_dev.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct amr_run amr_run;
+
+typedef struct {
+   int64_t timesteps_done;      /* timesteps completed so far */
+   int64_t stages_done;         /* stages completed so far */
+   int64_t block_stages;        /* sum over stages of active blocks */
+   double cell_var_updates;     /* block_stages * nx*ny*nz * num_vars */
+   int64_t active_blocks;       /* now */
+   int64_t max_slots;           /* high-water mark of used block slots */
+   double device_ms[4];         /* exchange, stencil, checksum, split/merge */
+   int nx, ny, nz, num_vars, stencil, precision;
+} amr_run_stats;
+
+amr_run *amr_run_create(int argc, char **argv);
+/* 0: more timesteps follow; 1: last timestep done; 2: stopped by the
+ * conservation check (the reference's early return). */
+int amr_run_step(amr_run *run);
+int amr_run_finish(amr_run *run);
+int amr_run_stats_get(const amr_run *run, amr_run_stats *out);
+void amr_run_report(amr_run *run);
+amr_dev *amr_run_device(amr_run *run);
+void amr_run_destroy(amr_run *run);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -15,6 +15,9 @@
 // partial sum (the checksum of ref/check_sum.c:36-65 fused into the
 // stencil epilogue); a second small kernel reduces the partials of each
 // variable in a fixed order.
+#include <stdlib.h>
+#include <string.h>
+
 #include "amr_internal.h"
 
 namespace {
@@ -102,6 +105,146 @@ __global__ __launch_bounds__(kThreads) void k_stencil(T *__restrict__ pool,
    }
 }
 
+// Plane-streaming stencil: one workgroup per (block, variable), one thread
+// per (j, k) column of the block.  The workgroup walks the x planes
+// 0..nx+1 once: each plane is prefetched into registers one step ahead,
+// parked in a 3-deep LDS ring, and every thread derives from it what its
+// column needs:
+//   27-point: the plane's 9-term sum S(p) at (j, k) in the reference's
+//             (j, k) order -- exactly the reference's sb/sm/sf for the
+//             outputs at x = p+1, p, p-1 -- so each output costs one 9-term
+//             sum, and w(i) = ((S(i-1) + S(i)) + S(i+1)) / 27 keeps the
+//             reference's association;
+//   7-point:  the four in-plane neighbours of plane i plus the column
+//             values of planes i-1 and i+1.
+// Output rows are written whole (the z-ghosts of the row are written back
+// unchanged from the staged copy) so every written cache line is full.
+template <typename T, int ST>
+__global__ __launch_bounds__(kThreads) void k_stencil_stream(T *__restrict__ pool,
+                                                              const int32_t *__restrict__ sorted,
+                                                              Geo g, int var0,
+                                                              double *__restrict__ partials,
+                                                              int part_stride)
+{
+   typedef typename Acc<T>::type A;
+   extern __shared__ __align__(16) unsigned char smem[];
+   const int plane = g.gy * g.gz;
+   A *ring = (A *) smem;                  // 3 planes
+   __shared__ double red[kThreads / 64];
+
+   const int in = blockIdx.x, var = var0 + blockIdx.y;
+   T *base = pool + (int64_t) var * g.var_stride + (int64_t) sorted[in] * g.slab;
+   const int tid = threadIdx.x;
+   const bool active = tid < g.ny * g.nz;
+   const int j = tid / g.nz + 1, k = tid - (j - 1) * g.nz + 1;
+   const int c = j * g.gz + k;            // column offset within a plane
+
+   // each thread stages at most kMaxPer elements of a plane
+   constexpr int kMaxPer = 4;
+   T pre[kMaxPer];
+#define PREFETCH(p)                                                        \
+   _Pragma("unroll") for (int q = 0; q < kMaxPer; q++) {                   \
+      const int e = tid + q * kThreads;                                    \
+      if (e < plane) pre[q] = base[(int64_t) (p) * plane + e];             \
+   }
+#define PARK(p)                                                            \
+   _Pragma("unroll") for (int q = 0; q < kMaxPer; q++) {                   \
+      const int e = tid + q * kThreads;                                    \
+      if (e < plane) ring[((p) % 3) * plane + e] = to_acc(pre[q]);         \
+   }
+
+   double bsum = 0.0;
+   PREFETCH(0);
+   PARK(0);
+   PREFETCH(1);
+   __syncthreads();
+   A s_prev = 0, s_cur = 0, c_prev = 0;
+   if (active) {
+      const A *pl = ring;
+      if (ST == 27) {
+         A a = pl[c - g.gz - 1] + pl[c - g.gz];
+         a = a + pl[c - g.gz + 1];
+         a = a + pl[c - 1];
+         a = a + pl[c];
+         a = a + pl[c + 1];
+         a = a + pl[c + g.gz - 1];
+         a = a + pl[c + g.gz];
+         a = a + pl[c + g.gz + 1];
+         s_prev = a;
+      } else {
+         c_prev = pl[c];
+      }
+   }
+   PARK(1);
+   if (2 <= g.nx + 1) { PREFETCH(2); }
+   __syncthreads();
+   if (active && ST == 27) {
+      const A *pl = ring + plane;
+      A a = pl[c - g.gz - 1] + pl[c - g.gz];
+      a = a + pl[c - g.gz + 1];
+      a = a + pl[c - 1];
+      a = a + pl[c];
+      a = a + pl[c + 1];
+      a = a + pl[c + g.gz - 1];
+      a = a + pl[c + g.gz];
+      a = a + pl[c + g.gz + 1];
+      s_cur = a;
+   }
+   for (int i = 1; i <= g.nx; i++) {
+      // plane i+1 into the ring slot of plane i-2 (last read two steps ago)
+      PARK(i + 1);
+      if (i + 2 <= g.nx + 1) { PREFETCH(i + 2); }
+      __syncthreads();
+      if (active) {
+         const A *pn = ring + ((i + 1) % 3) * plane;
+         A w;
+         if (ST == 27) {
+            A a = pn[c - g.gz - 1] + pn[c - g.gz];
+            a = a + pn[c - g.gz + 1];
+            a = a + pn[c - 1];
+            a = a + pn[c];
+            a = a + pn[c + 1];
+            a = a + pn[c + g.gz - 1];
+            a = a + pn[c + g.gz];
+            a = a + pn[c + g.gz + 1];
+            w = (s_prev + s_cur) + a;
+            w = w / A(27);
+            s_prev = s_cur;
+            s_cur = a;
+         } else {
+            const A *pc = ring + (i % 3) * plane;
+            const A cc = pc[c];
+            w = c_prev + pc[c - g.gz];
+            w = w + pc[c - 1];
+            w = w + cc;
+            w = w + pc[c + 1];
+            w = w + pc[c + g.gz];
+            w = w + pn[c];
+            w = w / A(7);
+            c_prev = cc;
+         }
+         const T out = from_acc<T>(w);
+         T *row = base + (int64_t) i * plane + j * g.gz;
+         row[k] = out;
+         // rewrite the row's z-ghosts unchanged so the row's lines are whole
+         const A *pc = ring + (i % 3) * plane;
+         if (k == 1)
+            row[0] = from_acc<T>(pc[c - 1]);
+         if (k == g.nz)
+            row[g.nz + 1] = from_acc<T>(pc[c + 1]);
+         if (partials)
+            bsum += (double) to_acc(out);
+      }
+   }
+#undef PREFETCH
+#undef PARK
+   if (partials) {
+      double s = block_reduce(bsum, red);
+      if (threadIdx.x == 0)
+         partials[(int64_t) var * part_stride + in] = s;
+   }
+}
+
 // Per-block interior sums of variables [var0, var0 + gridDim.y) without
 // touching the cells (stand-alone check_sum).
 template <typename T>
@@ -165,12 +308,37 @@ __global__ void k_convert_out(const T *__restrict__ pool, double *__restrict__ s
       dst[t] = (double) to_acc(src[t]);
 }
 
+// The plane-streaming kernel needs one thread per (j, k) column and at most
+// 4 staged elements per thread; larger block faces use the whole-slab kernel.
+static bool use_stream(const Geo &g)
+{
+   static int forced = -1;
+   if (forced < 0) {
+      const char *e = getenv("MINIAMR_STENCIL");
+      forced = (e && !strcmp(e, "slab")) ? 0 : 1;
+   }
+   return forced && g.ny * g.nz <= kThreads && g.gy * g.gz <= 4 * kThreads;
+}
+
 template <typename T>
 int launch_stencil(amr_dev *d, int start, int num, bool want_partials)
 {
    typedef typename Acc<T>::type A;
    if (!d->n_active || !num)
       return AMR_OK;
+   dim3 grid(d->n_active, num);
+   double *parts = want_partials ? d->d_partials : nullptr;
+   if (use_stream(d->g)) {
+      size_t lds = sizeof(A) * 3 * (size_t) d->g.gy * d->g.gz;
+      if (d->g.stencil == 7)
+         hipLaunchKernelGGL((k_stencil_stream<T, 7>), grid, dim3(kThreads), lds, d->stream,
+                            (T *) d->pool, d->d_sorted, d->g, start, parts, d->c.max_blocks);
+      else
+         hipLaunchKernelGGL((k_stencil_stream<T, 27>), grid, dim3(kThreads), lds, d->stream,
+                            (T *) d->pool, d->d_sorted, d->g, start, parts, d->c.max_blocks);
+      HIP_TRY(d, hipGetLastError());
+      return AMR_OK;
+   }
    size_t lds = sizeof(A) * (size_t) d->g.gx * d->g.gy * d->g.gz;
    static bool attr_set[2][2] = {{false, false}, {false, false}};
    const int si = d->g.stencil == 7 ? 0 : 1;
@@ -187,8 +355,6 @@ int launch_stencil(amr_dev *d, int start, int num, bool want_partials)
          attr_set[pi][si] = true;
       }
    }
-   dim3 grid(d->n_active, num);
-   double *parts = want_partials ? d->d_partials : nullptr;
    if (d->g.stencil == 7)
       hipLaunchKernelGGL((k_stencil<T, 7>), grid, dim3(kThreads), lds, d->stream,
                          (T *) d->pool, d->d_sorted, d->g, start, parts, d->c.max_blocks);

@@ -74,8 +74,17 @@ typedef struct {
 /* Growable job list for the device. */
 typedef struct { void *v; int n, cap, elem; } amr_vec;
 
+/* State of the timestep loop between amr_driver_step calls. */
 typedef struct {
+   int ts, comm_stage, state, n_rows;
+   double delta, sim_time, t_start;
+   double dev_ms[4];
+   double *sums;
+} amr_loop;
+
+typedef struct amr_run {
    amr_params P;
+   amr_loop loop;
 
    /* mesh */
    amr_block *blocks;          /* max_num_blocks slots */
@@ -155,6 +164,9 @@ void amr_consolidate_blocks(amr_run *R);
 
 /* driver.c */
 int amr_driver(amr_run *R);
+int amr_driver_begin(amr_run *R);
+int amr_driver_step(amr_run *R);
+void amr_driver_end(amr_run *R);
 double amr_calc_time_step(amr_run *R);
 void amr_report(amr_run *R);
 int amr_dump_final(amr_run *R);

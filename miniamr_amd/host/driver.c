@@ -143,10 +143,11 @@ static int check_timestep(amr_run *R, int ts, const double *sums, int64_t *block
    return 0;
 }
 
-/* ref/driver.c:37-165 */
-int amr_driver(amr_run *R)
+/* ref/driver.c:37-63: initial mesh, values and refinement. */
+int amr_driver_begin(amr_run *R)
 {
    amr_params *P = &R->P;
+   amr_loop *L = &R->loop;
    const char *sp = getenv("AMR_DUMP_SUMS");
    g_sum_log = sp ? fopen(sp, "w") : NULL;
 
@@ -164,80 +165,115 @@ int amr_driver(amr_run *R)
    R->t_refine_all += t2 - t1;
    R->nb_min = R->nb_max = R->global_active;
 
-   int n_rows = P->checksum_freq ? (P->stages_per_ts + P->checksum_freq - 1) / P->checksum_freq : 0;
-   double *sums = (double *) amr_xmalloc(sizeof(double) * (size_t) (n_rows ? n_rows : 1) *
-                                         P->num_vars);
-   amr_dev_check(R, amr_dev_reserve_sums(R->dev, n_rows ? n_rows : 1), "reserve_sums");
+   L->n_rows = P->checksum_freq ? (P->stages_per_ts + P->checksum_freq - 1) / P->checksum_freq : 0;
+   L->sums = (double *) amr_xmalloc(sizeof(double) * (size_t) (L->n_rows ? L->n_rows : 1) *
+                                    P->num_vars);
+   amr_dev_check(R, amr_dev_reserve_sums(R->dev, L->n_rows ? L->n_rows : 1), "reserve_sums");
    amr_dev_set_timing(R->dev, 1);
-   amr_dev_read_timing(R->dev, (double[4]){0});
+   double ms[4];
+   amr_dev_read_timing(R->dev, ms);
 
-   double delta = 1.0, sim_time = 0.0, dev_ms[4] = { 0, 0, 0, 0 };
-   if (P->use_time)
-      delta = amr_calc_time_step(R);
-   int comm_stage = 0, done = 0, ts;
-   int failed = 0;
-   for (ts = 1; !done; ts++) {
-      if (P->report_perf & 8)
-         printf("Timestep %d time %lf delta %lf\n", ts, sim_time, delta);
-      enqueue_timestep(R, comm_stage);
-      comm_stage += P->stages_per_ts;
-      if (n_rows)
-         amr_dev_check(R, amr_dev_read_sums(R->dev, 0, n_rows, sums), "read_sums");
-      else
-         amr_dev_check(R, amr_dev_sync(R->dev), "sync");
-      double t3 = amr_now();
-      double ms[4];
-      amr_dev_read_timing(R->dev, ms);
-      for (int i = 0; i < 4; i++)
-         dev_ms[i] += ms[i];
-      if (check_timestep(R, ts, sums, &R->total_blocks)) {
-         failed = 1;
-         break;
-      }
-      double t4 = amr_now();
-      R->t_cs += t4 - t3;
+   L->delta = P->use_time ? amr_calc_time_step(R) : 1.0;
+   L->sim_time = 0.0;
+   L->comm_stage = 0;
+   L->ts = 1;
+   L->state = 0;
+   L->t_start = t1;
+   return 0;
+}
 
-      if (P->num_refine && !P->uniform_refine) {
-         amr_move_objects(R, delta);
-         if (!(ts % P->refine_freq)) {
-            amr_refine(R, ts);
-            if (R->global_active < R->nb_min)
-               R->nb_min = R->global_active;
-            if (R->global_active > R->nb_max)
-               R->nb_max = R->global_active;
-         }
-      }
+/* One timestep of ref/driver.c:67-160.  Returns 0 while more timesteps
+ * follow, 1 after the last one, 2 if a conservation check stopped the run. */
+int amr_driver_step(amr_run *R)
+{
+   amr_params *P = &R->P;
+   amr_loop *L = &R->loop;
+   if (L->state)
+      return L->state;
+   int ts = L->ts;
+   if (P->report_perf & 8)
+      printf("Timestep %d time %lf delta %lf\n", ts, L->sim_time, L->delta);
+   enqueue_timestep(R, L->comm_stage);
+   L->comm_stage += P->stages_per_ts;
+   if (L->n_rows)
+      amr_dev_check(R, amr_dev_read_sums(R->dev, 0, L->n_rows, L->sums), "read_sums");
+   else
       amr_dev_check(R, amr_dev_sync(R->dev), "sync");
-      R->t_refine_all += amr_now() - t4;
-
-      sim_time += delta;
-      if (P->use_time) {
-         if (P->use_tsteps && ts >= P->num_tsteps)
-            done = 1;
-         delta = amr_calc_time_step(R);
-         if (sim_time >= P->end_time)
-            done = 1;
-      } else if (ts >= P->num_tsteps) {
-         done = 1;
-      }
-   }
+   double t3 = amr_now();
    double ms[4];
    amr_dev_read_timing(R->dev, ms);
    for (int i = 0; i < 4; i++)
-      dev_ms[i] += ms[i];
-   R->t_comm = dev_ms[0] * 1e-3;
-   R->t_calc = dev_ms[1] * 1e-3;
-   R->t_cs += dev_ms[2] * 1e-3;
-   if (!failed) {
-      P->end_time = sim_time;
-      P->num_tsteps = ts - 1;
-      R->t_all = amr_now() - t1;
+      L->dev_ms[i] += ms[i];
+   if (check_timestep(R, ts, L->sums, &R->total_blocks)) {
+      L->state = 2;
+      return 2;
    }
-   free(sums);
+   double t4 = amr_now();
+   R->t_cs += t4 - t3;
+
+   if (P->num_refine && !P->uniform_refine) {
+      amr_move_objects(R, L->delta);
+      if (!(ts % P->refine_freq)) {
+         amr_refine(R, ts);
+         if (R->global_active < R->nb_min)
+            R->nb_min = R->global_active;
+         if (R->global_active > R->nb_max)
+            R->nb_max = R->global_active;
+      }
+   }
+   amr_dev_check(R, amr_dev_sync(R->dev), "sync");
+   R->t_refine_all += amr_now() - t4;
+
+   L->sim_time += L->delta;
+   int done = 0;
+   if (P->use_time) {
+      if (P->use_tsteps && ts >= P->num_tsteps)
+         done = 1;
+      L->delta = amr_calc_time_step(R);
+      if (L->sim_time >= P->end_time)
+         done = 1;
+   } else if (ts >= P->num_tsteps) {
+      done = 1;
+   }
+   L->ts++;
+   if (done)
+      L->state = 1;
+   return L->state;
+}
+
+/* ref/driver.c:162-164 and the timer bookkeeping for the report. */
+void amr_driver_end(amr_run *R)
+{
+   amr_params *P = &R->P;
+   amr_loop *L = &R->loop;
+   double ms[4];
+   amr_dev_read_timing(R->dev, ms);
+   for (int i = 0; i < 4; i++)
+      L->dev_ms[i] += ms[i];
+   R->t_comm = L->dev_ms[0] * 1e-3;
+   R->t_calc = L->dev_ms[1] * 1e-3;
+   R->t_cs += L->dev_ms[2] * 1e-3;
+   if (L->state != 2) {
+      P->end_time = L->sim_time;
+      P->num_tsteps = L->ts - 1;
+      R->t_all = amr_now() - L->t_start;
+   }
+   free(L->sums);
+   L->sums = NULL;
    if (g_sum_log)
       fclose(g_sum_log);
    g_sum_log = NULL;
-   return failed;
+}
+
+/* ref/driver.c:37-165 */
+int amr_driver(amr_run *R)
+{
+   int rc;
+   amr_driver_begin(R);
+   while (!(rc = amr_driver_step(R)))
+      ;
+   amr_driver_end(R);
+   return rc == 2;
 }
 
 /* Summary line of ref/profile.c:1298 plus this build's throughput line. */

@@ -1,0 +1,71 @@
+/*
+ * run_api.c -- include/amr_run.h: the driver of main.c, one timestep per
+ * call, for callers that time or drive the run themselves.
+ */
+#include <stdlib.h>
+#include <string.h>
+
+#include "amr.h"
+#include "amr_run.h"
+
+amr_run *amr_run_create(int argc, char **argv)
+{
+   amr_run *R = (amr_run *) amr_xmalloc(sizeof(amr_run));
+   amr_parse_args(&R->P, argc, argv);
+   if (amr_check_input(&R->P))
+      exit(-1);
+   if (!R->P.block_change)
+      R->P.block_change = R->P.num_refine;
+   amr_allocate(R);
+   amr_driver_begin(R);
+   return R;
+}
+
+int amr_run_step(amr_run *R)
+{
+   return amr_driver_step(R);
+}
+
+int amr_run_finish(amr_run *R)
+{
+   amr_driver_end(R);
+   return R->loop.state == 2;
+}
+
+int amr_run_stats_get(const amr_run *R, amr_run_stats *out)
+{
+   const amr_params *P = &R->P;
+   memset(out, 0, sizeof(*out));
+   out->timesteps_done = R->loop.ts - 1;
+   out->stages_done = out->timesteps_done * P->stages_per_ts;
+   out->block_stages = R->total_blocks;
+   out->cell_var_updates = (double) R->total_blocks * P->nx * P->ny * P->nz * P->num_vars;
+   out->active_blocks = amr_n_active(R);
+   out->max_slots = R->max_active_block;
+   memcpy(out->device_ms, R->loop.dev_ms, sizeof(out->device_ms));
+   out->nx = P->nx; out->ny = P->ny; out->nz = P->nz;
+   out->num_vars = P->num_vars;
+   out->stencil = P->stencil;
+   out->precision = P->precision;
+   return 0;
+}
+
+void amr_run_report(amr_run *R)
+{
+   amr_report(R);
+   fflush(stdout);
+}
+
+amr_dev *amr_run_device(amr_run *R)
+{
+   return R->dev;
+}
+
+void amr_run_destroy(amr_run *R)
+{
+   if (!R)
+      return;
+   amr_dump_final(R);
+   amr_free(R);
+   free(R);
+}
