@@ -49,7 +49,8 @@ class Stats(ctypes.Structure):
     _fields_ = [("timesteps_done", ctypes.c_int64), ("stages_done", ctypes.c_int64),
                 ("block_stages", ctypes.c_int64), ("cell_var_updates", ctypes.c_double),
                 ("active_blocks", ctypes.c_int64), ("max_slots", ctypes.c_int64),
-                ("device_ms", ctypes.c_double * 4), ("nx", ctypes.c_int), ("ny", ctypes.c_int),
+                ("device_ms", ctypes.c_double * 4), ("stage_calls", ctypes.c_int64),
+                ("nx", ctypes.c_int), ("ny", ctypes.c_int),
                 ("nz", ctypes.c_int), ("num_vars", ctypes.c_int), ("stencil", ctypes.c_int),
                 ("precision", ctypes.c_int)]
 
@@ -154,7 +155,7 @@ def main():
     value = updates / elapsed / 1e9
     ebytes = PREC_BYTES[s1.precision]
     stencil_s = (s1.device_ms[1] - s0.device_ms[1]) * 1e-3
-    launches = (s1.stages_done - s0.stages_done) * (1 if a.schedule == "stage" else 40)
+    launches = s1.stage_calls - s0.stage_calls
     achieved = updates * 2 * ebytes / stencil_s / 1e9 if stencil_s > 0 else None
     traffic = None
     pmc = os.path.join(REPO, "profiles", "stencil27_cfg1_%s_pmc.json" % a.precision)

@@ -39,6 +39,7 @@ typedef struct {
    int64_t active_blocks;       /* now */
    int64_t max_slots;           /* high-water mark of used block slots */
    double device_ms[4];         /* exchange, stencil, checksum, split/merge */
+   int64_t stage_calls;         /* amr_dev_stage calls (one stencil launch each) */
    int nx, ny, nz, num_vars, stencil, precision;
 } amr_run_stats;
 

@@ -1,15 +1,29 @@
-// amr_internal.h -- device context and helpers shared by the HIP sources of
-// libminiamr_dev.so (gfx950).  Not part of the public ABI (include/amr_dev.h).
+// amr_internal.h -- device context and cell-record layout shared by the
+// HIP sources of libminiamr_dev.so (gfx950).  Not part of the public ABI.
 //
-// Cell pool layout in HBM: one allocation, structure of arrays
-//   pool[var][slot][x][y][z]     (z fastest, one ghost layer per side)
-// Each (var, slot) slab holds gx*gy*gz values (g* = n* + 2) padded to a
-// multiple of 256 bytes, so every slab starts on its own 256-byte boundary
-// and one variable's slabs are contiguous: a stage that works on one
-// variable streams one dense region.  The pool is zeroed at creation and
-// slots are never cleared afterwards, because ghost cells that the exchange
-// does not write keep whatever the slot held before -- the 27-point stencil
-// reads some of them, exactly as the reference does.
+// Storage.  A block's cells are the reference's (nx+2)(ny+2)(nz+2) array
+// with one ghost layer (ref/block.h:52), but in HBM each (variable, slot)
+// owns one *record* whose parts are each contiguous:
+//   I    interior [nx][ny][nz]                      (z fastest)
+//   F0..F5 face ghosts, interior of the face only:  x faces [ny][nz],
+//        y faces [nx][nz], z faces [nx][ny]
+//   EXY  the 4 ghost lines along z where x and y are both ghosts,
+//        [xs][ys][0..nz+1] (includes the 8 corners)
+//   EXZ  the 4 ghost lines along y, [xs][zs][1..ny]
+//   EYZ  the 4 ghost lines along x, [ys][zs][1..nx]
+//   ZI, ZX, ZY  a copy of the cells at z = 1 and z = nz (zs = 0, 1):
+//        ZI [zs][nx][ny] interior, ZX [zs][xs][0..ny+1] (x ghosts),
+//        ZY [zs][ys][1..nx] (y ghosts).
+// Every canonical cell (i, j, k) has exactly one home (home()); the Z*
+// copies exist because z is the fastest axis of I: the z-direction ghost
+// exchange reads other blocks' z = 1 / z = nz planes, which in I are one
+// element per cache line.  Every writer of a cell at z = 1 or z = nz also
+// writes its copy (store_cell()), so the copies always equal their homes.
+// With this layout every access of the exchange and of the stencil is a
+// contiguous run; the pool is [var][slot][record], zeroed at creation and
+// never cleared, since ghost cells the exchange does not write keep what
+// the slot held before (the 27-point stencil reads some of them, exactly
+// as in the reference).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -17,17 +31,20 @@
 #include <vector>
 #include "amr_dev.h"
 
-struct Geo {
+struct Lay {
    int nx, ny, nz;          // interior sizes
    int gx, gy, gz;          // with ghosts
-   int64_t slab;            // elements per (var, slot), padded
-   int64_t var_stride;      // elements per variable = slab * max_blocks
    int stencil;
+   int oF[6];               // face planes
+   int oEXY, oEXZ, oEYZ;    // edge lines
+   int oZI, oZX, oZY;       // z = 1 / z = nz copies
+   int64_t rec;             // elements per (var, slot), padded to 256 bytes
+   int64_t var_stride;      // rec * max_blocks
 };
 
 struct amr_dev {
    amr_dev_config c;
-   Geo g;
+   Lay g;
    int elem_bytes;
    void *pool = nullptr;
    hipStream_t stream = nullptr;
@@ -41,7 +58,7 @@ struct amr_dev {
    double *d_partials = nullptr;    // [num_vars][max_blocks] per-block sums
    double *d_sums = nullptr;        // checksum table [rows][num_vars]
    int sum_rows = 0;
-   double *h_pinned = nullptr;      // small pinned buffer for read-backs
+   double *h_pinned = nullptr;      // pinned buffer for read-backs
    size_t h_pinned_bytes = 0;
 
    double *d_stage = nullptr;       // one slot in FP64 for upload/download
@@ -83,11 +100,43 @@ template <> __device__ __forceinline__ uint16_t from_acc<uint16_t>(float v)
    return (uint16_t) (u >> 16);
 }
 
-__device__ __forceinline__ int64_t cell_index(const Geo &g, int var, int slot, int i, int j,
-                                              int k)
+// Offset of canonical cell (i, j, k), 0 <= i <= nx+1 etc., in a record.
+__host__ __device__ __forceinline__ int home(const Lay &L, int i, int j, int k)
 {
-   return (int64_t) var * g.var_stride + (int64_t) slot * g.slab +
-          ((int64_t) i * g.gy + j) * g.gz + k;
+   const int gi = (i == 0) | (i == L.nx + 1), gj = (j == 0) | (j == L.ny + 1),
+             gk = (k == 0) | (k == L.nz + 1);
+   const int xs = i != 0, ys = j != 0, zs = k != 0;
+   switch (gi + gj + gk) {
+   case 0:
+      return ((i - 1) * L.ny + (j - 1)) * L.nz + (k - 1);
+   case 1:
+      if (gi) return L.oF[xs] + (j - 1) * L.nz + (k - 1);
+      if (gj) return L.oF[2 + ys] + (i - 1) * L.nz + (k - 1);
+      return L.oF[4 + zs] + (i - 1) * L.ny + (j - 1);
+   default:
+      if (gi && gj) return L.oEXY + (xs * 2 + ys) * L.gz + k;
+      if (gi) return L.oEXZ + (xs * 2 + zs) * L.ny + (j - 1);
+      return L.oEYZ + (ys * 2 + zs) * L.nx + (i - 1);
+   }
+}
+
+// Offset of the z-plane copy of cell (i, j, k) with k == 1 or k == nz.
+__host__ __device__ __forceinline__ int zcopy(const Lay &L, int i, int j, int k)
+{
+   const int zs = k == L.nz;
+   const int gi = (i == 0) | (i == L.nx + 1), gj = (j == 0) | (j == L.ny + 1);
+   if (!gi && !gj) return L.oZI + (zs * L.nx + (i - 1)) * L.ny + (j - 1);
+   if (gi) return L.oZX + (zs * 2 + (i != 0)) * L.gy + j;
+   return L.oZY + (zs * 2 + (j != 0)) * L.nx + (i - 1);
+}
+
+// Write a cell and, on the z = 1 / z = nz planes, its copy.
+template <typename T>
+__device__ __forceinline__ void store_cell(T *rec, const Lay &L, int i, int j, int k, T v)
+{
+   rec[home(L, i, j, k)] = v;
+   if (k == 1 || k == L.nz)
+      rec[zcopy(L, i, j, k)] = v;
 }
 
 // Error helpers (context.hip).

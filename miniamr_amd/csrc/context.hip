@@ -94,15 +94,28 @@ int amr_dev_create(const amr_dev_config *cfg, amr_dev **out)
       return amr_set_error(d, AMR_ERR_NODEV, "device %d is %s, this library is built for gfx950",
                            cfg->device, prop.gcnArchName);
 
-   Geo &g = d->g;
+   Lay &g = d->g;
    g.nx = cfg->nx; g.ny = cfg->ny; g.nz = cfg->nz;
    g.gx = g.nx + 2; g.gy = g.ny + 2; g.gz = g.nz + 2;
    g.stencil = cfg->stencil;
+   int o = g.nx * g.ny * g.nz;                       // interior first
+   g.oF[0] = o; o += g.ny * g.nz;
+   g.oF[1] = o; o += g.ny * g.nz;
+   g.oF[2] = o; o += g.nx * g.nz;
+   g.oF[3] = o; o += g.nx * g.nz;
+   g.oF[4] = o; o += g.nx * g.ny;
+   g.oF[5] = o; o += g.nx * g.ny;
+   g.oEXY = o; o += 4 * g.gz;
+   g.oEXZ = o; o += 4 * g.ny;
+   g.oEYZ = o; o += 4 * g.nx;
+   g.oZI = o; o += 2 * g.nx * g.ny;
+   g.oZX = o; o += 4 * g.gy;
+   g.oZY = o; o += 4 * g.nx;
    d->elem_bytes = cfg->precision == AMR_PREC_F64 ? 8 : cfg->precision == AMR_PREC_F32 ? 4 : 2;
    int64_t per256 = 256 / d->elem_bytes;
    int64_t cells = (int64_t) g.gx * g.gy * g.gz;
-   g.slab = (cells + per256 - 1) / per256 * per256;
-   g.var_stride = g.slab * cfg->max_blocks;
+   g.rec = ((int64_t) o + per256 - 1) / per256 * per256;
+   g.var_stride = g.rec * cfg->max_blocks;
    size_t pool_bytes = (size_t) g.var_stride * cfg->num_vars * d->elem_bytes;
 
    HIP_TRY(d, hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));

@@ -15,83 +15,88 @@
 // stencil y faces extend along x and z faces along x and y so that edges
 // and corners fill transitively (ref/comm.c:1496-1533), and boundary
 // planes cover the whole ghost plane (ref/comm.c:1941-1964).
+//
+// Cells are addressed canonically (i, j, k) and mapped to the record
+// layout (amr_internal.h); the z phase reads source planes z = 1 / z = nz
+// from their contiguous copies.
 #include "amr_internal.h"
 
 namespace {
 
-struct FaceFrame {
-   int64_t sp, sa, sb;   // strides of the normal axis and the two in-plane axes
-   int n, na, nb;        // sizes along those axes (interior)
-};
-
-__device__ __forceinline__ FaceFrame frame(const Geo &g, int dir)
+// canonical coordinates of in-plane point (a, b) on plane p of direction dir
+__device__ __forceinline__ void canon(int dir, int p, int a, int b, int &i, int &j, int &k)
 {
-   FaceFrame f;
-   const int64_t sx = (int64_t) g.gy * g.gz, sy = g.gz, sz = 1;
-   if (dir == 0) {        // normal x, in-plane (y, z)
-      f.sp = sx; f.sa = sy; f.sb = sz; f.n = g.nx; f.na = g.ny; f.nb = g.nz;
-   } else if (dir == 1) { // normal y, in-plane (x, z)
-      f.sp = sy; f.sa = sx; f.sb = sz; f.n = g.ny; f.na = g.nx; f.nb = g.nz;
-   } else {               // normal z, in-plane (x, y)
-      f.sp = sz; f.sa = sx; f.sb = sy; f.n = g.nz; f.na = g.nx; f.nb = g.ny;
-   }
-   return f;
+   if (dir == 0) { i = p; j = a; k = b; }
+   else if (dir == 1) { i = a; j = p; k = b; }
+   else { i = a; j = b; k = p; }
+}
+
+template <typename T>
+__device__ __forceinline__ T load_src(const T *rec, const Lay &L, int dir, int p, int a, int b)
+{
+   int i, j, k;
+   canon(dir, p, a, b, i, j, k);
+   return rec[dir == 2 ? zcopy(L, i, j, k) : home(L, i, j, k)];
 }
 
 template <typename T>
 __global__ __launch_bounds__(128) void k_exchange(T *__restrict__ pool,
                                                   const amr_face_job *__restrict__ jobs,
-                                                  Geo g, int var0)
+                                                  Lay L, int var0)
 {
    typedef typename Acc<T>::type A;
-   const amr_face_job j = jobs[blockIdx.x];
+   const amr_face_job jb = jobs[blockIdx.x];
    const int var = var0 + blockIdx.y;
-   const int dir = j.face >> 1, hi = j.face & 1;
-   const FaceFrame f = frame(g, dir);
-   const int ghost = hi ? f.n + 1 : 0;
-   const int plane = hi ? 1 : f.n;      // neighbour plane next to the shared face
-   T *dst = pool + (int64_t) var * g.var_stride + (int64_t) j.dst * g.slab + ghost * f.sp;
-   const T *src = pool + (int64_t) var * g.var_stride + (int64_t) j.src * g.slab;
+   const int dir = jb.face >> 1, hi = jb.face & 1;
+   const int n = dir == 0 ? L.nx : dir == 1 ? L.ny : L.nz;
+   const int na = dir == 0 ? L.ny : L.nx;   // slower in-plane axis
+   const int nb = dir == 2 ? L.ny : L.nz;   // faster in-plane axis
+   const int ghost = hi ? n + 1 : 0;
+   const int plane = hi ? 1 : n;            // neighbour plane next to the shared face
+   T *dst = pool + (int64_t) var * L.var_stride + (int64_t) jb.dst * L.rec;
+   const T *src = pool + (int64_t) var * L.var_stride + (int64_t) jb.src * L.rec;
 
-   if (j.kind == AMR_FACE_SAME || j.kind == AMR_FACE_BC) {
+   if (jb.kind == AMR_FACE_SAME || jb.kind == AMR_FACE_BC) {
       int ext_a = 0, ext_b = 0;
-      if (g.stencil == 27) {
-         if (j.kind == AMR_FACE_BC) ext_a = ext_b = 1;
+      if (L.stencil == 27) {
+         if (jb.kind == AMR_FACE_BC) ext_a = ext_b = 1;
          else if (dir == 1) ext_a = 1;
          else if (dir == 2) ext_a = ext_b = 1;
       }
       const int a0 = 1 - ext_a, b0 = 1 - ext_b;
-      const int wa = f.na + 2 * ext_a, wb = f.nb + 2 * ext_b;
+      const int wa = na + 2 * ext_a, wb = nb + 2 * ext_b;
       // BC reads the block's own plane next to the face
-      const int sp = j.kind == AMR_FACE_BC ? (hi ? f.n : 1) : plane;
-      const T *s = src + sp * f.sp;
+      const int sp = jb.kind == AMR_FACE_BC ? (hi ? n : 1) : plane;
       for (int t = threadIdx.x; t < wa * wb; t += blockDim.x) {
          const int a = a0 + t / wb, b = b0 + t % wb;
-         const int64_t o = a * f.sa + b * f.sb;
-         dst[o] = s[o];
+         const T v = load_src(src, L, dir, sp, a, b);
+         int i, j, k;
+         canon(dir, ghost, a, b, i, j, k);
+         store_cell(dst, L, i, j, k, v);
       }
-   } else if (j.kind == AMR_FACE_PROLONG) {
+   } else if (jb.kind == AMR_FACE_PROLONG) {
       // fine cell (a, b) of the whole face <- coarse cell of the quarter
-      const int ao = j.jq * (f.na / 2), bo = j.iq * (f.nb / 2);
-      const T *s = src + plane * f.sp;
-      for (int t = threadIdx.x; t < f.na * f.nb; t += blockDim.x) {
-         const int a = 1 + t / f.nb, b = 1 + t % f.nb;
+      const int ao = jb.jq * (na / 2), bo = jb.iq * (nb / 2);
+      for (int t = threadIdx.x; t < na * nb; t += blockDim.x) {
+         const int a = 1 + t / nb, b = 1 + t % nb;
          const int ca = ((a + 1) >> 1) + ao, cb = ((b + 1) >> 1) + bo;
-         A v = to_acc(s[ca * f.sa + cb * f.sb]) / A(4);
-         dst[a * f.sa + b * f.sb] = from_acc<T>(v);
+         const A v = to_acc(load_src(src, L, dir, plane, ca, cb)) / A(4);
+         int i, j, k;
+         canon(dir, ghost, a, b, i, j, k);
+         store_cell(dst, L, i, j, k, from_acc<T>(v));
       }
    } else {  // RESTRICT: coarse quarter cell <- sum of 4 fine cells
-      const int ha = f.na / 2, hb = f.nb / 2;
-      const int ao = j.jq * ha, bo = j.iq * hb;
-      const T *s = src + plane * f.sp;
+      const int ha = na / 2, hb = nb / 2;
+      const int ao = jb.jq * ha, bo = jb.iq * hb;
       for (int t = threadIdx.x; t < ha * hb; t += blockDim.x) {
          const int a = 1 + t / hb, b = 1 + t % hb;
-         const int64_t o00 = (2 * a - 1) * f.sa + (2 * b - 1) * f.sb;
-         A v = to_acc(s[o00]);
-         v = v + to_acc(s[o00 + f.sb]);
-         v = v + to_acc(s[o00 + f.sa]);
-         v = v + to_acc(s[o00 + f.sa + f.sb]);
-         dst[(a + ao) * f.sa + (b + bo) * f.sb] = from_acc<T>(v);
+         A v = to_acc(load_src(src, L, dir, plane, 2 * a - 1, 2 * b - 1));
+         v = v + to_acc(load_src(src, L, dir, plane, 2 * a - 1, 2 * b));
+         v = v + to_acc(load_src(src, L, dir, plane, 2 * a, 2 * b - 1));
+         v = v + to_acc(load_src(src, L, dir, plane, 2 * a, 2 * b));
+         int i, j, k;
+         canon(dir, ghost, a + ao, b + bo, i, j, k);
+         store_cell(dst, L, i, j, k, from_acc<T>(v));
       }
    }
 }

@@ -4,8 +4,8 @@
 // cell it lies in divided by 8.  consolidate (ref/block.c:411-431): every
 // cell of the merged block's octant o is the sum of the 2x2x2 child cells
 // it covers, added in the reference's order (x fastest, then y, then z).
-// Only interior cells are written; a new block's ghost cells keep what the
-// slot held before, as in the reference.
+// Only interior cells (and their z-plane copies) are written; a new block's
+// ghost cells keep what the slot held before, as in the reference.
 //
 // A batch is applied "as if one job after another": a slot freed by an
 // earlier job (a split parent, a consolidated child) can be the target of
@@ -19,53 +19,53 @@ namespace {
 // Snapshot of parent octants: scratch[job][var][hx][hy][hz].
 template <typename T>
 __global__ void k_split_snap(const T *__restrict__ pool, const amr_split_job *__restrict__ jobs,
-                             T *__restrict__ scratch, Geo g, int nvars)
+                             T *__restrict__ scratch, Lay g, int nvars)
 {
    const amr_split_job j = jobs[blockIdx.x];
    const int var = blockIdx.y;
    const int hx = g.nx / 2, hy = g.ny / 2, hz = g.nz / 2, hc = hx * hy * hz;
    const int i1 = (j.octant & 1) * hx, j1 = ((j.octant >> 1) & 1) * hy, k1 = (j.octant >> 2) * hz;
-   const T *p = pool + (int64_t) var * g.var_stride + (int64_t) j.parent * g.slab;
+   const T *p = pool + (int64_t) var * g.var_stride + (int64_t) j.parent * g.rec;
    T *s = scratch + ((int64_t) blockIdx.x * nvars + var) * hc;
    for (int t = threadIdx.x; t < hc; t += blockDim.x) {
       const int i = t / (hy * hz), r = t - i * hy * hz, jj = r / hz, k = r - jj * hz;
-      s[t] = p[((int64_t) (i + 1 + i1) * g.gy + (jj + 1 + j1)) * g.gz + (k + 1 + k1)];
+      s[t] = p[((i + i1) * g.ny + (jj + j1)) * g.nz + (k + k1)];
    }
 }
 
 template <typename T>
 __global__ void k_split_write(T *__restrict__ pool, const amr_split_job *__restrict__ jobs,
-                              const T *__restrict__ scratch, Geo g, int nvars)
+                              const T *__restrict__ scratch, Lay g, int nvars)
 {
    typedef typename Acc<T>::type A;
    const amr_split_job j = jobs[blockIdx.x];
    const int var = blockIdx.y;
    const int hy = g.ny / 2, hz = g.nz / 2, hc = (g.nx / 2) * hy * hz;
-   T *c = pool + (int64_t) var * g.var_stride + (int64_t) j.child * g.slab;
+   T *c = pool + (int64_t) var * g.var_stride + (int64_t) j.child * g.rec;
    const T *s = scratch + ((int64_t) blockIdx.x * nvars + var) * hc;
    const int nyz = g.ny * g.nz, ncell = g.nx * nyz;
    for (int t = threadIdx.x; t < ncell; t += blockDim.x) {
       const int i = t / nyz, r = t - i * nyz, jj = r / g.nz, k = r - jj * g.nz;
       const A v = to_acc(s[((i >> 1) * hy + (jj >> 1)) * hz + (k >> 1)]) / A(8);
-      c[((int64_t) (i + 1) * g.gy + (jj + 1)) * g.gz + (k + 1)] = from_acc<T>(v);
+      store_cell(c, g, i + 1, jj + 1, k + 1, from_acc<T>(v));
    }
 }
 
 // Merged sums: scratch[job][var][hx][hy][hz].
 template <typename T>
 __global__ void k_merge_sum(const T *__restrict__ pool, const amr_merge_job *__restrict__ jobs,
-                            T *__restrict__ scratch, Geo g, int nvars)
+                            T *__restrict__ scratch, Lay g, int nvars)
 {
    typedef typename Acc<T>::type A;
    const amr_merge_job j = jobs[blockIdx.x];
    const int var = blockIdx.y;
    const int hy = g.ny / 2, hz = g.nz / 2, hc = (g.nx / 2) * hy * hz;
-   const T *c = pool + (int64_t) var * g.var_stride + (int64_t) j.child * g.slab;
+   const T *c = pool + (int64_t) var * g.var_stride + (int64_t) j.child * g.rec;
    T *s = scratch + ((int64_t) blockIdx.x * nvars + var) * hc;
-   const int64_t sx = (int64_t) g.gy * g.gz, sy = g.gz;
+   const int sx = g.ny * g.nz, sy = g.nz;
    for (int t = threadIdx.x; t < hc; t += blockDim.x) {
       const int i = t / (hy * hz), r = t - i * hy * hz, jj = r / hz, k = r - jj * hz;
-      const int64_t o = (2 * i + 1) * sx + (2 * jj + 1) * sy + (2 * k + 1);
+      const int o = (2 * i) * sx + (2 * jj) * sy + (2 * k);
       A v = to_acc(c[o]);
       v = v + to_acc(c[o + sx]);
       v = v + to_acc(c[o + sy]);
@@ -80,17 +80,17 @@ __global__ void k_merge_sum(const T *__restrict__ pool, const amr_merge_job *__r
 
 template <typename T>
 __global__ void k_merge_write(T *__restrict__ pool, const amr_merge_job *__restrict__ jobs,
-                              const T *__restrict__ scratch, Geo g, int nvars)
+                              const T *__restrict__ scratch, Lay g, int nvars)
 {
    const amr_merge_job j = jobs[blockIdx.x];
    const int var = blockIdx.y;
    const int hx = g.nx / 2, hy = g.ny / 2, hz = g.nz / 2, hc = hx * hy * hz;
    const int i1 = (j.octant & 1) * hx, j1 = ((j.octant >> 1) & 1) * hy, k1 = (j.octant >> 2) * hz;
-   T *p = pool + (int64_t) var * g.var_stride + (int64_t) j.dst * g.slab;
+   T *p = pool + (int64_t) var * g.var_stride + (int64_t) j.dst * g.rec;
    const T *s = scratch + ((int64_t) blockIdx.x * nvars + var) * hc;
    for (int t = threadIdx.x; t < hc; t += blockDim.x) {
       const int i = t / (hy * hz), r = t - i * hy * hz, jj = r / hz, k = r - jj * hz;
-      p[((int64_t) (i + 1 + i1) * g.gy + (jj + 1 + j1)) * g.gz + (k + 1 + k1)] = s[t];
+      store_cell(p, g, i + 1 + i1, jj + 1 + j1, k + 1 + k1, s[t]);
    }
 }
 

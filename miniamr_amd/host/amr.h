@@ -79,6 +79,7 @@ typedef struct {
    int ts, comm_stage, state, n_rows;
    double delta, sim_time, t_start;
    double dev_ms[4];
+   int64_t stage_calls;
    double *sums;
 } amr_loop;
 

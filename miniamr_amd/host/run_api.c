@@ -43,6 +43,7 @@ int amr_run_stats_get(const amr_run *R, amr_run_stats *out)
    out->active_blocks = amr_n_active(R);
    out->max_slots = R->max_active_block;
    memcpy(out->device_ms, R->loop.dev_ms, sizeof(out->device_ms));
+   out->stage_calls = R->loop.stage_calls;
    out->nx = P->nx; out->ny = P->ny; out->nz = P->nz;
    out->num_vars = P->num_vars;
    out->stencil = P->stencil;
