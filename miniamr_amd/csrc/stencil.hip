@@ -4,9 +4,10 @@
 // Stencil: one workgroup per (active block, variable), one thread per
 // (j, k) column (up to kCols columns per thread).  The workgroup walks the
 // x planes 0..nx+1 once: each canonical plane is gathered from the record
-// (interior rows, face planes, edge lines) into registers one step ahead,
-// parked in a 3-deep LDS ring, and every thread derives from it what its
-// columns need:
+// (interior rows, face planes, edge lines) into registers kDepth planes
+// ahead (so several planes of loads are in flight per workgroup), parked in
+// a 3-deep LDS ring, and every thread derives from it what its columns
+// need:
 //   27-point: the plane's 9-term sum S(p) at (j, k), summed left to right
 //             over (j-1..j+1) x (k-1..k+1) -- exactly the reference's
 //             sb/sm/sf for the outputs at x = p+1, p, p-1
@@ -32,6 +33,7 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kCols = 4;       // columns per thread: ny*nz <= 1024
 constexpr int kPer = 8;        // plane elements per thread: gy*gz <= 2048
+constexpr int kDepth = 4;      // planes of loads in flight per workgroup
 
 __device__ __forceinline__ double block_reduce(double v, double *red)
 {
@@ -63,120 +65,154 @@ __device__ __forceinline__ A sum9(const A *pl, int c, int gz)
    return a;
 }
 
-template <typename T, int ST, int NCOL, int NPER>
+// Division of the 27- and 7-point sums.  FP64: IEEE division, so the path
+// is bit-identical to the reference.  FP32/BF16 forms: division by a known
+// constant with one FMA correction of the reciprocal product (Markstein's
+// method for a fixed divisor): q = x*r, e = fma(-q, d, x), q' = fma(e, r, q)
+// with r = RN(1/d).  Unlike a bare x*r it carries no systematic bias (RN(1/7)
+// alone is 4.5e-8 high, which would drift conserved sums by ~1e-5 over 400
+// stages); on 2e6 random operands it agreed with IEEE division everywhere.
+template <typename A, int DIV>
+__device__ __forceinline__ A divide(A w)
+{
+   if (sizeof(A) == 8)
+      return w / A(DIV);
+   const float r = 1.0f / (float) DIV;
+   const float q = (float) w * r;
+   const float e = __fmaf_rn(-q, (float) DIV, (float) w);
+   return (A) __fmaf_rn(e, r, q);
+}
+
+template <typename T, int ST, int NCOL, int NPER, int VPW>
 __global__ __launch_bounds__(kThreads) void k_stencil(T *__restrict__ pool,
                                                        const int32_t *__restrict__ sorted,
-                                                       Lay L, int var0,
+                                                       Lay L, int var0, int var_end,
                                                        double *__restrict__ partials,
                                                        int part_stride)
 {
    typedef typename Acc<T>::type A;
+   constexpr int D = kDepth;           // planes of global loads in flight
    extern __shared__ __align__(16) unsigned char smem[];
-   const int plane = L.gy * L.gz, ncol = L.ny * L.nz;
-   A *ring = (A *) smem;
+   const int plane = L.gy * L.gz, ncol = L.ny * L.nz, nplanes = L.nx + 2;
+   A *ring = (A *) smem;               // per variable: 3 staged planes
    __shared__ double red[kThreads / 64];
 
-   const int in = blockIdx.x, var = var0 + blockIdx.y;
-   T *rec = pool + (int64_t) var * L.var_stride + (int64_t) sorted[in] * L.rec;
+   const int in = blockIdx.x, v0 = var0 + blockIdx.y * VPW;
+   const int nv = min(VPW, var_end - v0);
+   T *rec = pool + (int64_t) v0 * L.var_stride + (int64_t) sorted[in] * L.rec;
+   const int64_t vs = L.var_stride;
    const int tid = threadIdx.x;
 
-   // record offsets of this thread's plane elements, per plane kind
-   // (offsets inside a plane depend on p only through the plane's base)
-   T pre[NPER];
-   auto prefetch = [&](int p) {
+   // Record offsets of this thread's plane elements: plane 0, plane nx+1,
+   // and interior planes p, where the offset is affine in p.
+   int off0[NPER], offN[NPER], off1[NPER], stp[NPER];
 #pragma unroll
-      for (int q = 0; q < NPER; q++) {
-         const int e = tid + q * kThreads;
-         if (e < plane) {
-            const int j = e / L.gz, k = e - j * L.gz;
-            pre[q] = rec[home(L, p, j, k)];
-         }
-      }
-   };
-   auto park = [&](int p) {
-#pragma unroll
-      for (int q = 0; q < NPER; q++) {
-         const int e = tid + q * kThreads;
-         if (e < plane)
-            ring[(p % 3) * plane + e] = to_acc(pre[q]);
-      }
-   };
+   for (int q = 0; q < NPER; q++) {
+      const int e = min(tid + q * kThreads, plane - 1);
+      const int jj = e / L.gz, kk = e - jj * L.gz;
+      off0[q] = home(L, 0, jj, kk);
+      offN[q] = home(L, L.nx + 1, jj, kk);
+      off1[q] = home(L, 1, jj, kk);
+      stp[q] = L.nx > 1 ? home(L, 2, jj, kk) - off1[q] : 0;
+   }
+   T pre[D][VPW][NPER];
+#define PREFETCH(slot, p)                                                  \
+   _Pragma("unroll") for (int q = 0; q < NPER; q++) {                      \
+      if (tid + q * kThreads < plane) {                                    \
+         const int o = (p) == 0 ? off0[q] : (p) == L.nx + 1 ? offN[q]       \
+                                         : off1[q] + ((p) - 1) * stp[q];    \
+         _Pragma("unroll") for (int u = 0; u < VPW; u++)                   \
+            if (u < nv) pre[slot][u][q] = rec[u * vs + o];                 \
+      }                                                                    \
+   }
 
-   A s_prev[NCOL], s_cur[NCOL], c_prev[NCOL];
-   double bsum = 0.0;
-   prefetch(0);
-   park(0);
-   prefetch(1);
-   __syncthreads();
+   A s_prev[VPW][NCOL], s_cur[VPW][NCOL], c_prev[VPW][NCOL];
+   double bsum[VPW];
 #pragma unroll
-   for (int q = 0; q < NCOL; q++) {
-      const int col = tid + q * kThreads;
-      if (col < ncol) {
-         const int j = col / L.nz + 1, k = col - (j - 1) * L.nz + 1, c = j * L.gz + k;
-         if (ST == 27)
-            s_prev[q] = sum9(ring, c, L.gz);
-         else
-            c_prev[q] = ring[c];
-      }
-   }
-   park(1);
-   if (2 <= L.nx + 1)
-      prefetch(2);
-   __syncthreads();
-   if (ST == 27) {
+   for (int u = 0; u < VPW; u++)
+      bsum[u] = 0.0;
 #pragma unroll
-      for (int q = 0; q < NCOL; q++) {
-         const int col = tid + q * kThreads;
-         if (col < ncol) {
-            const int j = col / L.nz + 1, k = col - (j - 1) * L.nz + 1, c = j * L.gz + k;
-            s_cur[q] = sum9(ring + plane, c, L.gz);
+   for (int d = 0; d < D; d++)
+      if (d < nplanes) { PREFETCH(d, d); }
+
+   for (int p0 = 0; p0 < nplanes; p0 += D) {
+#pragma unroll
+      for (int d = 0; d < D; d++) {
+         const int p = p0 + d;
+         if (p >= nplanes)
+            break;
+         // park plane p (ring slot p % 3 was last read two planes ago)
+#pragma unroll
+         for (int u = 0; u < VPW; u++) {
+            A *slot = ring + (u * 3 + p % 3) * plane;
+#pragma unroll
+            for (int q = 0; q < NPER; q++) {
+               const int e = tid + q * kThreads;
+               if (e < plane && u < nv)
+                  slot[e] = to_acc(pre[d][u][q]);
+            }
+         }
+         if (p + D < nplanes) { PREFETCH(d, p + D); }
+         __syncthreads();
+#pragma unroll
+         for (int u = 0; u < VPW; u++) {
+            if (u >= nv)
+               continue;
+            const A *slot = ring + (u * 3 + p % 3) * plane;
+            const A *pc = ring + (u * 3 + (p + 2) % 3) * plane;   // plane p - 1
+            T *ru = rec + u * vs;
+#pragma unroll
+            for (int q = 0; q < NCOL; q++) {
+               const int col = tid + q * kThreads;
+               if (col >= ncol)
+                  continue;
+               const int j = col / L.nz + 1, k = col - (j - 1) * L.nz + 1, c = j * L.gz + k;
+               A w = 0;
+               bool out_now = p >= 2;
+               if (ST == 27) {
+                  const A s_new = sum9(slot, c, L.gz);
+                  if (out_now) {
+                     w = (s_prev[u][q] + s_cur[u][q]) + s_new;
+                     w = divide<A, 27>(w);
+                  }
+                  s_prev[u][q] = s_cur[u][q];
+                  s_cur[u][q] = s_new;
+               } else {
+                  if (out_now) {
+                     const A cc = pc[c];
+                     w = c_prev[u][q] + pc[c - L.gz];
+                     w = w + pc[c - 1];
+                     w = w + cc;
+                     w = w + pc[c + 1];
+                     w = w + pc[c + L.gz];
+                     w = w + slot[c];
+                     w = divide<A, 7>(w);
+                     c_prev[u][q] = cc;
+                  } else if (p == 0) {
+                     c_prev[u][q] = slot[c];
+                  }
+               }
+               if (out_now) {
+                  const T out = from_acc<T>(w);
+                  const int i = p - 1;
+                  ru[((i - 1) * L.ny + (j - 1)) * L.nz + (k - 1)] = out;
+                  if (k == 1 || k == L.nz)
+                     ru[zcopy(L, i, j, k)] = out;
+                  if (partials)
+                     bsum[u] += (double) to_acc(out);
+               }
+            }
          }
       }
    }
-   for (int i = 1; i <= L.nx; i++) {
-      // plane i+1 into the ring slot of plane i-2 (last read two steps ago)
-      park(i + 1);
-      if (i + 2 <= L.nx + 1)
-         prefetch(i + 2);
-      __syncthreads();
-      const A *pc = ring + (i % 3) * plane;
-      const A *pn = ring + ((i + 1) % 3) * plane;
-#pragma unroll
-      for (int q = 0; q < NCOL; q++) {
-         const int col = tid + q * kThreads;
-         if (col >= ncol)
-            continue;
-         const int j = col / L.nz + 1, k = col - (j - 1) * L.nz + 1, c = j * L.gz + k;
-         A w;
-         if (ST == 27) {
-            const A s_next = sum9(pn, c, L.gz);
-            w = (s_prev[q] + s_cur[q]) + s_next;
-            w = w / A(27);
-            s_prev[q] = s_cur[q];
-            s_cur[q] = s_next;
-         } else {
-            const A cc = pc[c];
-            w = c_prev[q] + pc[c - L.gz];
-            w = w + pc[c - 1];
-            w = w + cc;
-            w = w + pc[c + 1];
-            w = w + pc[c + L.gz];
-            w = w + pn[c];
-            w = w / A(7);
-            c_prev[q] = cc;
-         }
-         const T out = from_acc<T>(w);
-         rec[((i - 1) * L.ny + (j - 1)) * L.nz + (k - 1)] = out;
-         if (k == 1 || k == L.nz)
-            rec[zcopy(L, i, j, k)] = out;
-         if (partials)
-            bsum += (double) to_acc(out);
-      }
-   }
+#undef PREFETCH
    if (partials) {
-      double s = block_reduce(bsum, red);
-      if (threadIdx.x == 0)
-         partials[(int64_t) var * part_stride + in] = s;
+#pragma unroll
+      for (int u = 0; u < VPW; u++) {
+         double s = block_reduce(bsum[u], red);
+         if (threadIdx.x == 0 && u < nv)
+            partials[(int64_t) (v0 + u) * part_stride + in] = s;
+      }
    }
 }
 
@@ -244,15 +280,23 @@ __global__ void k_convert_out(const T *__restrict__ pool, double *__restrict__ s
    }
 }
 
+// Variables per workgroup: two for the narrow storage types (the per-cell
+// index work and barriers are shared), one for FP64.
+template <typename T> struct Vpw { static constexpr int v = 2; };
+template <> struct Vpw<double> { static constexpr int v = 1; };
+
 template <typename T, int ST>
-int launch_st(amr_dev *d, dim3 grid, double *parts, int start)
+int launch_st(amr_dev *d, int start, int num, double *parts)
 {
    const Lay &L = d->g;
+   constexpr int V = Vpw<T>::v;
    const int ncol = L.ny * L.nz, plane = L.gy * L.gz;
-   size_t lds = sizeof(typename Acc<T>::type) * 3 * (size_t) plane;
-#define GO(NC, NP)                                                                        \
-   hipLaunchKernelGGL((k_stencil<T, ST, NC, NP>), grid, dim3(kThreads), lds, d->stream,   \
-                      (T *) d->pool, d->d_sorted, L, start, parts, d->c.max_blocks)
+   dim3 grid(d->n_active, (num + V - 1) / V);
+   size_t lds = sizeof(typename Acc<T>::type) * 3 * V * (size_t) plane;
+#define GO(NC, NP)                                                                          \
+   hipLaunchKernelGGL((k_stencil<T, ST, NC, NP, V>), grid, dim3(kThreads), lds, d->stream,  \
+                      (T *) d->pool, d->d_sorted, L, start, start + num, parts,             \
+                      d->c.max_blocks)
    if (ncol <= kThreads && plane <= 2 * kThreads)
       GO(1, 2);
    else if (ncol <= 2 * kThreads && plane <= 4 * kThreads)
@@ -272,10 +316,9 @@ int launch_stencil(amr_dev *d, int start, int num, bool want_partials)
 {
    if (!d->n_active || !num)
       return AMR_OK;
-   dim3 grid(d->n_active, num);
    double *parts = want_partials ? d->d_partials : nullptr;
-   return d->g.stencil == 7 ? launch_st<T, 7>(d, grid, parts, start)
-                            : launch_st<T, 27>(d, grid, parts, start);
+   return d->g.stencil == 7 ? launch_st<T, 7>(d, start, num, parts)
+                            : launch_st<T, 27>(d, start, num, parts);
 }
 
 template <typename T>
