@@ -108,6 +108,42 @@ def cpu_baseline(timeout=240):
                       % (threads, bpts, updates, t, calc, updates / calc / 1e9)}
 
 
+def run_workload(lib, precision, schedule, warmup, steps, sums_path=None):
+    """One full run of the workload; times timesteps warmup+1 .. warmup+steps."""
+    os.environ["MINIAMR_PRECISION"] = precision
+    os.environ["MINIAMR_SCHEDULE"] = schedule
+    if sums_path:
+        os.environ["AMR_DUMP_SUMS"] = sums_path
+    else:
+        os.environ.pop("AMR_DUMP_SUMS", None)
+    argv = [b"miniAMR.x"] + [x.encode() for x in CFG1.split()] + [b"--report_perf", b"0"]
+    arr = (ctypes.c_char_p * len(argv))(*argv)
+    t_setup = time.time()
+    run = lib.amr_run_create(len(argv), arr)
+    for _ in range(warmup):
+        lib.amr_run_step(run)
+    s0 = stats(lib, run)
+    setup_s = time.time() - t_setup
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        if lib.amr_run_step(run) == 2:
+            raise SystemExit("conservation check stopped the run")
+    t1 = time.perf_counter()   # amr_run_step returns after the device is idle
+    s1 = stats(lib, run)
+    # finish the remaining timesteps so every checksum of the workload is logged
+    while lib.amr_run_step(run) == 0:
+        pass
+    lib.amr_run_finish(run)
+    lib.amr_run_destroy(run)
+    os.environ.pop("AMR_DUMP_SUMS", None)
+    return {"s0": s0, "s1": s1, "elapsed": t1 - t0, "setup_s": setup_s}
+
+
+def read_sums(path):
+    with open(path) as f:
+        return [float(x) for x in f.read().split()]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,39 +154,24 @@ def main():
     ap.add_argument("--schedule", default=os.environ.get("MINIAMR_SCHEDULE", "stage"),
                     choices=["stage", "var"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-precision-report", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build runs one rank on one GPU; --gpus %d is not supported "
               "(the multi-GPU decomposition is a design, see DESIGN.md)" % a.gpus,
               file=sys.stderr)
         sys.exit(2)
-    if a.warmup + a.steps > 10:
-        print("bench.py: the workload has 10 timesteps; warmup + steps must be <= 10",
-              file=sys.stderr)
+    if a.warmup + a.steps > 10 or a.steps < 1:
+        print("bench.py: the workload has 10 timesteps; need 1 <= steps and "
+              "warmup + steps <= 10", file=sys.stderr)
         sys.exit(2)
-    os.environ["MINIAMR_PRECISION"] = a.precision
-    os.environ["MINIAMR_SCHEDULE"] = a.schedule
 
     lib = load_host()
-    argv = [b"miniAMR.x"] + [x.encode() for x in CFG1.split()] + [b"--report_perf", b"0"]
-    arr = (ctypes.c_char_p * len(argv))(*argv)
-    t_setup = time.time()
-    run = lib.amr_run_create(len(argv), arr)
-    for _ in range(a.warmup):
-        lib.amr_run_step(run)
-    s0 = stats(lib, run)
-    setup_s = time.time() - t_setup
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        rc = lib.amr_run_step(run)
-        if rc == 2:
-            raise SystemExit("conservation check stopped the run")
-    t1 = time.perf_counter()   # amr_run_step returns after the device is idle
-    s1 = stats(lib, run)
-    lib.amr_run_finish(run)
-    lib.amr_run_destroy(run)
-
-    elapsed = t1 - t0
+    import tempfile
+    tmp = tempfile.mkdtemp(prefix="miniamr_bench_")
+    main_sums = os.path.join(tmp, "sums_%s.txt" % a.precision)
+    r = run_workload(lib, a.precision, a.schedule, a.warmup, a.steps, main_sums)
+    s0, s1, elapsed = r["s0"], r["s1"], r["elapsed"]
     updates = s1.cell_var_updates - s0.cell_var_updates
     value = updates / elapsed / 1e9
     ebytes = PREC_BYTES[s1.precision]
@@ -161,7 +182,30 @@ def main():
     pmc = os.path.join(REPO, "profiles", "stencil27_cfg1_%s_pmc.json" % a.precision)
     if os.path.exists(pmc):
         with open(pmc) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            per_bv = json.load(f).get("hbm_bytes_per_block_variable")
+        if per_bv and launches:
+            block_vars = (s1.block_stages - s0.block_stages) * s1.num_vars / launches
+            traffic = round(per_bv * block_vars)
+    avg_blocks = (s1.block_stages - s0.block_stages) / max(1, s1.stages_done - s0.stages_done)
+
+    precision_form = None
+    if not a.no_precision_report and a.precision == "f64":
+        ref = read_sums(main_sums)
+        precision_form = {}
+        for prec in ("f32", "bf16"):
+            path = os.path.join(tmp, "sums_%s.txt" % prec)
+            q = run_workload(lib, prec, a.schedule, a.warmup, a.steps, path)
+            got = read_sums(path)
+            n = min(len(ref), len(got))
+            errs = [abs(g - f) / abs(f) for g, f in zip(got[:n], ref[:n])]
+            upd = q["s1"].cell_var_updates - q["s0"].cell_var_updates
+            v = upd / q["elapsed"] / 1e9
+            precision_form[prec] = {
+                "value": round(v, 4), "ratio_vs_f64": round(v / value, 3),
+                "checksums_compared": n,
+                "max_checksum_rel_err_vs_f64": float("%.3e" % max(errs)) if errs else None,
+                "final_event_max_rel_err": float("%.3e" % max(errs[-40:])) if errs else None}
+
     line = {
         "metric": "GCell-var updates/s",
         "value": round(value, 4),
@@ -179,8 +223,7 @@ def main():
                                "num_refine 4, 2 moving objects, timesteps %d-%d of 10"
                                % (a.warmup + 1, a.warmup + a.steps),
                    "schedule": a.schedule,
-                   "avg_active_blocks": round((s1.block_stages - s0.block_stages) /
-                                              max(1, s1.stages_done - s0.stages_done), 1),
+                   "avg_active_blocks": round(avg_blocks, 1),
                    "cell_var_updates": updates},
         "roofline": {"bound": "hbm",
                      "achieved": round(achieved, 1) if achieved else None,
@@ -195,7 +238,8 @@ def main():
                       "stencil": round(s1.device_ms[1] - s0.device_ms[1], 2),
                       "checksum": round(s1.device_ms[2] - s0.device_ms[2], 2),
                       "split_merge": round(s1.device_ms[3] - s0.device_ms[3], 2)},
-        "setup_s": round(setup_s, 2),
+        "setup_s": round(r["setup_s"], 2),
+        "precision_form": precision_form,
         "cpu_baseline": None if a.no_cpu_baseline else cpu_baseline(),
     }
     print(json.dumps(line), flush=True)
