@@ -88,6 +88,14 @@ CASES = {
                 "--stencil 27 --num_refine 2 --num_tsteps 3 --stages_per_ts 5 "
                 "--refine_freq 1 " + SPHERE0, "cells"),
     "err_flag": ("--nx 4 --bogus 3", None),
+    # The bandwidth-scale workload (27-point, 4 levels, two objects).  The
+    # serial reference needs ~20 minutes for it, so it is only generated
+    # when named explicitly; per-block sums are kept for 3 variables.
+    "cfg1": ("--init_x 4 --init_y 4 --init_z 4 --nx 16 --ny 16 --nz 16 --num_vars 40 "
+             "--stencil 27 --num_tsteps 10 --stages_per_ts 20 --checksum_freq 5 --num_refine 4 "
+             "--refine_freq 4 --max_blocks 8000 --error_tol -10 --num_objects 2 "
+             "--object 2 0 0.15 0.15 0.15 0.02 0.02 0.02 0.1 0.1 0.1 0 0 0 "
+             "--object 0 0 0.5 0.5 0.5 0.02 0.02 0.02 0.05 0.05 0.05 0 0 0", "bsums_only"),
     "err_odd": ("--nx 5 --num_tsteps 1", None),
 }
 
@@ -138,6 +146,8 @@ def run_case(name, args, kind, outdir=HERE):
             env["AMR_DUMP_CELLS"] = os.path.join(td, "cells.bin")
         elif kind == "bsums":
             env["AMR_DUMP_CELLS"] = os.path.join(td, "cells.bin")
+            env["AMR_DUMP_BSUMS"] = os.path.join(td, "bsums.bin")
+        elif kind == "bsums_only":
             env["AMR_DUMP_BSUMS"] = os.path.join(td, "bsums.bin")
         p = subprocess.run([REF_DUMP] + args.split(), cwd=td, env=env,
                            stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
@@ -193,15 +203,30 @@ def run_case(name, args, kind, outdir=HERE):
                 sample_block=bi.astype(np.int32), sample_var=vi.astype(np.int32),
                 sample_idx=np.stack([ii, jj, kk], axis=1).astype(np.int32),
                 sample_val=vals)
+        if kind == "bsums_only" and os.path.exists(env["AMR_DUMP_BSUMS"]):
+            save_bsums_subset(name, read_dump(env["AMR_DUMP_BSUMS"]), outdir)
         with open(os.path.join(outdir, name + ".json"), "w") as f:
             json.dump(rec, f, indent=0)
         return rec
 
 
+BSUM_VARS = [0, 19, 39]
+
+
+def save_bsums_subset(name, d, outdir=HERE):
+    np.savez_compressed(
+        os.path.join(outdir, name + ".bsums.npz"),
+        number=np.array([b[0] for b in d["blocks"]], dtype=np.int64),
+        level=np.array([b[1] for b in d["blocks"]], dtype=np.int32),
+        slot=np.array([b[2] for b in d["blocks"]], dtype=np.int32),
+        vars=np.array(BSUM_VARS, dtype=np.int32),
+        sums=np.stack([b[3][BSUM_VARS] for b in d["blocks"]]))
+
+
 def main(names):
     if not os.path.exists(REF_DUMP):
         sys.exit("build the reference first: make -C oracle ref")
-    for name in names or list(CASES):
+    for name in names or [c for c in CASES if CASES[c][1] != "bsums_only"]:
         args, kind = CASES[name]
         rec = run_case(name, args, kind)
         print("%-12s rc=%d sums=%d counts=%d msgs=%s" % (
