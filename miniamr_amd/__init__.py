@@ -81,12 +81,148 @@ def load_library(path=LIB_PATH):
         "amr_dev_sync": (ip, [vp]),
         "amr_dev_set_timing": (ip, [vp, ip]),
         "amr_dev_read_timing": (ip, [vp, dp]),
+        # include/amr_batch.h, device half
+        "amr_batch_create": (ip, [ip, ip, ip, ctypes.POINTER(vp)]),
+        "amr_batch_destroy": (ip, [vp]),
+        "amr_batch_last_error": (ctypes.c_char_p, [vp]),
+        "amr_batch_add": (ip, [vp, ctypes.POINTER(Config), ctypes.POINTER(vp)]),
+        "amr_batch_set_active": (ip, [vp, vp, ip]),
+        "amr_batch_set_shape": (ip, [vp, ip, ip, ip]),
+        "amr_batch_get_shape": (ip, [vp, ctypes.POINTER(ctypes.c_int)]),
+        "amr_batch_stage": (ip, [vp, ip, ip, ip]),
+        "amr_batch_timestep": (ip, [vp, ip, ip, ip, ip]),
+        "amr_batch_autotune": (ip, [vp, dp]),
+        "amr_batch_sync": (ip, [vp]),
+        "amr_batch_set_timing": (ip, [vp, ip]),
+        "amr_batch_read_timing": (ip, [vp, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
         f.restype = res
         f.argtypes = args
     return lib
+
+
+class RunStats(ctypes.Structure):
+    """amr_run_stats of include/amr_run.h."""
+    _fields_ = [("timesteps_done", ctypes.c_int64), ("stages_done", ctypes.c_int64),
+                ("block_stages", ctypes.c_int64), ("cell_var_updates", ctypes.c_double),
+                ("active_blocks", ctypes.c_int64), ("max_slots", ctypes.c_int64),
+                ("device_ms", ctypes.c_double * 4), ("stage_calls", ctypes.c_int64),
+                ("nx", ctypes.c_int), ("ny", ctypes.c_int),
+                ("nz", ctypes.c_int), ("num_vars", ctypes.c_int), ("stencil", ctypes.c_int),
+                ("precision", ctypes.c_int)]
+
+
+HOST_LIB_PATH = os.path.join(PKG, "lib", "libminiamr_host.so")
+
+
+def load_host_library(path=HOST_LIB_PATH):
+    """Load the driver library (include/amr_run.h and the driver half of
+    include/amr_batch.h)."""
+    if not os.path.exists(path):
+        raise AmrError("%s is not built: run `make -C miniamr_amd`" % path)
+    lib = ctypes.CDLL(path)
+    vp, ip, dp = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    argv_t = ctypes.POINTER(ctypes.c_char_p)
+    sig = {
+        "amr_run_create": (vp, [ip, argv_t]),
+        "amr_run_step": (ip, [vp]),
+        "amr_run_finish": (ip, [vp]),
+        "amr_run_stats_get": (ip, [vp, ctypes.POINTER(RunStats)]),
+        "amr_run_report": (None, [vp]),
+        "amr_run_device": (vp, [vp]),
+        "amr_run_destroy": (None, [vp]),
+        "amr_batch_run_create": (vp, [ip, ctypes.POINTER(ctypes.c_int),
+                                      ctypes.POINTER(argv_t), ip]),
+        "amr_batch_run_step": (ip, [vp]),
+        "amr_batch_run_finish": (ip, [vp]),
+        "amr_batch_run_instances": (ip, [vp]),
+        "amr_batch_run_stats_get": (ip, [vp, ip, ctypes.POINTER(RunStats)]),
+        "amr_batch_run_get": (vp, [vp, ip]),
+        "amr_batch_run_batch": (vp, [vp]),
+        "amr_batch_run_tune_ms": (dp, [vp]),
+        "amr_batch_run_device_ms": (dp, [vp]),
+        "amr_batch_run_destroy": (None, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+def _argv(args, prog=b"miniAMR.x"):
+    toks = [prog] + [x.encode() for x in (args.split() if isinstance(args, str) else args)]
+    arr = (ctypes.c_char_p * (len(toks) + 1))(*toks, None)
+    return len(toks), arr
+
+
+class Run:
+    """One drop-in driver run (include/amr_run.h), stepped from Python."""
+
+    def __init__(self, args, host=None):
+        self.lib = host or load_host_library()
+        self._argc, self._argv = _argv(args)
+        self.h = self.lib.amr_run_create(self._argc, self._argv)
+
+    def step(self):
+        return self.lib.amr_run_step(self.h)
+
+    def stats(self):
+        st = RunStats()
+        self.lib.amr_run_stats_get(self.h, ctypes.byref(st))
+        return st
+
+    def finish(self):
+        return self.lib.amr_run_finish(self.h)
+
+    def close(self):
+        if self.h:
+            self.lib.amr_run_destroy(self.h)
+            self.h = None
+
+
+class BatchRun:
+    """Many drop-in driver runs stepped together on one GPU
+    (include/amr_batch.h, driver half)."""
+
+    def __init__(self, runs, autotune=True, host=None):
+        self.lib = host or load_host_library()
+        self._keep = [_argv(r, b"miniAMR_batch.x") for r in runs]
+        n = len(runs)
+        argcs = (ctypes.c_int * n)(*[k[0] for k in self._keep])
+        argvs = (ctypes.POINTER(ctypes.c_char_p) * n)(
+            *[ctypes.cast(k[1], ctypes.POINTER(ctypes.c_char_p)) for k in self._keep])
+        self.n = n
+        self.h = self.lib.amr_batch_run_create(n, argcs, argvs, 1 if autotune else 0)
+
+    def step(self):
+        return self.lib.amr_batch_run_step(self.h)
+
+    def stats(self, i):
+        st = RunStats()
+        self.lib.amr_batch_run_stats_get(self.h, i, ctypes.byref(st))
+        return st
+
+    def tune_ms(self):
+        return self.lib.amr_batch_run_tune_ms(self.h)
+
+    def device_ms(self):
+        return self.lib.amr_batch_run_device_ms(self.h)
+
+    def shape(self, devlib):
+        sh = (ctypes.c_int * 3)()
+        devlib.amr_batch_get_shape(self.lib.amr_batch_run_batch(self.h), sh)
+        return list(sh)
+
+    def finish(self):
+        return self.lib.amr_batch_run_finish(self.h)
+
+    def close(self):
+        if self.h:
+            self.lib.amr_batch_run_destroy(self.h)
+            self.h = None
 
 
 def _ptr(a, ctype):

@@ -48,6 +48,8 @@ struct amr_dev {
    int elem_bytes;
    void *pool = nullptr;
    hipStream_t stream = nullptr;
+   bool owns_stream = true;         // false for batch members (batch's stream)
+   int plan_version = 0;            // bumped when device plan buffers change
 
    int32_t *d_sorted = nullptr;     // active slots in sorted-list order
    int n_active = 0;

@@ -1,0 +1,742 @@
+// batch.hip -- many independent instances stepped together
+// (include/amr_batch.h).
+//
+// Work lists.  Every member is an ordinary context (its own pool, layout,
+// sorted slot list, face jobs, partials and checksum table).  For each
+// kernel of a stage the batch keeps a flat list of work items over all
+// active members, stored implicitly: members are ordered by block shape
+// (largest first) and a prefix table pre[m] gives the first item of member
+// m, so item t belongs to the member m with pre[m] <= t < pre[m+1] (binary
+// search on a wave-uniform value) and its local index addresses the
+// member's own data:
+//   exchange, direction d: (var group g, face job j), local = g*count[d] + j
+//   stencil:               (var group g, sorted block in), local = g*n_active + in
+//   checksum reduce:       variable v, local = v
+// One wavefront computes one item (wave-level synchronisation, its own LDS
+// ring), and the kernels are persistent: the waves of a grid of
+// `wgs` workgroups stride through the list.  Small blocks therefore no
+// longer leave three quarters of a 256-thread workgroup idle, and one
+// launch per stage phase serves every member.
+//
+// Arithmetic.  Items run exactly the bodies of the single-instance kernels
+// (kern_stencil.h, kern_exchange.h), so cells are bit-identical to running
+// each member alone; only the association of the per-block checksum
+// partials and of their sum differs (FP64, a few ulp).
+//
+// Graphs.  amr_batch_timestep captures a timestep's stages (3 exchange
+// launches, a stencil launch and, on checksum stages, a reduce launch per
+// stage) into one hipGraph, reused until the work lists or launch shapes
+// change.  No memset or copy node is recorded.
+//
+// Autotuner.  amr_batch_autotune snapshots the members' pools, times
+// candidate shapes (variables per stencil item; persistent grid sizes) on
+// the live meshes with HIP events, keeps the fastest, and restores the
+// pools, so tuning never changes results.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+#include <algorithm>
+#include <vector>
+
+#include "amr_batch.h"
+#include "kern_exchange.h"
+#include "kern_stencil.h"
+
+namespace {
+
+using namespace amrk;
+
+constexpr int kWavesPerWg = 4;   // workgroups of 256 threads, one item per wave
+
+// One member as the kernels see it.
+struct BMember {
+   void *pool;
+   const amr_face_job *jobs;
+   const int32_t *sorted;
+   double *partials;
+   double *sums;
+   Lay L;
+   int n_active, num_vars, part_stride, cls;
+   int count[3], joff[3];
+};
+
+// Shape classes of the wave-per-item stencil: columns (ny*nz) per lane and
+// plane elements ((ny+2)*(nz+2)) per lane over 64 lanes.
+//   0: ny*nz <= 64,  plane <= 128   (e.g. 8^3 blocks)
+//   1: ny*nz <= 192, plane <= 256   (e.g. 10^3, 12^3)
+//   2: ny*nz <= 256, plane <= 384   (e.g. 14^3, 16^3)
+int shape_class(const Lay &L)
+{
+   const int ncol = L.ny * L.nz, plane = L.gy * L.gz;
+   if (ncol <= 64 && plane <= 128) return 0;
+   if (ncol <= 192 && plane <= 256) return 1;
+   if (ncol <= 256 && plane <= 384) return 2;
+   return -1;
+}
+
+__device__ __forceinline__ int find_member(const int *__restrict__ pre, int M, int t)
+{
+   int lo = 0, hi = M;   // pre[lo] <= t < pre[hi]
+   while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (pre[mid] <= t) lo = mid;
+      else hi = mid;
+   }
+   return lo;
+}
+
+__device__ __forceinline__ int first_item()
+{
+   return __builtin_amdgcn_readfirstlane((int) (blockIdx.x * kWavesPerWg + (threadIdx.x >> 6)));
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void kb_exchange(const BMember *__restrict__ mem,
+                                                   const int *__restrict__ pre, int M, int dir)
+{
+   const int lane = threadIdx.x & 63;
+   const int total = pre[M], stride = gridDim.x * kWavesPerWg;
+   for (int t = first_item(); t < total; t += stride) {
+      const int m = find_member(pre, M, t);
+      const BMember &b = mem[m];
+      const int local = t - pre[m], cnt = b.count[dir];
+      const int g = local / cnt, j = local - g * cnt;
+      const amr_face_job jb = b.jobs[b.joff[dir] + j];
+      const int v0 = g * kVars;
+      exchange_job<T>((T *) b.pool, jb, b.L, v0, min(kVars, b.num_vars - v0), lane, 64);
+   }
+}
+
+template <typename T, int ST, int VPW>
+__global__ __launch_bounds__(256) void kb_stencil(const BMember *__restrict__ mem,
+                                                  const int *__restrict__ pre, int M,
+                                                  int want_partials, int ring_elems)
+{
+   typedef typename Acc<T>::type A;
+   extern __shared__ __align__(16) unsigned char smem[];
+   A *ring = (A *) smem + (threadIdx.x >> 6) * ring_elems;
+   const int total = pre[M], stride = gridDim.x * kWavesPerWg;
+   for (int t = first_item(); t < total; t += stride) {
+      const int m = find_member(pre, M, t);
+      const BMember &b = mem[m];
+      const int na = b.n_active, local = t - pre[m];
+      const int g = local / na, in = local - g * na;
+      const int v0 = g * VPW, nv = min(VPW, b.num_vars - v0);
+      const Lay L = b.L;
+      T *rec = (T *) b.pool + (int64_t) v0 * L.var_stride + (int64_t) b.sorted[in] * L.rec;
+      double *part = want_partials ? b.partials + (int64_t) v0 * b.part_stride + in : nullptr;
+      switch (b.cls) {
+      case 0:
+         stencil_block<T, ST, 1, 2, VPW, 64>(rec, L, nv, ring, nullptr, part, b.part_stride);
+         break;
+      case 1:
+         stencil_block<T, ST, 3, 4, VPW, 64>(rec, L, nv, ring, nullptr, part, b.part_stride);
+         break;
+      default:
+         stencil_block<T, ST, 4, 6, VPW, 64>(rec, L, nv, ring, nullptr, part, b.part_stride);
+         break;
+      }
+   }
+}
+
+// Checksum of variable v of a member: the per-block partials summed in
+// sorted-block order per lane (blocks lane, lane+64, ...), then a fixed
+// shuffle tree.
+__global__ __launch_bounds__(256) void kb_reduce(const BMember *__restrict__ mem,
+                                                 const int *__restrict__ pre, int M, int row)
+{
+   const int lane = threadIdx.x & 63;
+   const int total = pre[M], stride = gridDim.x * kWavesPerWg;
+   for (int t = first_item(); t < total; t += stride) {
+      const int m = find_member(pre, M, t);
+      const BMember &b = mem[m];
+      const int v = t - pre[m];
+      const double *p = b.partials + (int64_t) v * b.part_stride;
+      double s = 0.0;
+      for (int i = lane; i < b.n_active; i += 64)
+         s += p[i];
+      s = wave_reduce(s);
+      if (lane == 0)
+         b.sums[(int64_t) row * b.num_vars + v] = s;
+   }
+}
+
+}  // namespace
+
+struct amr_batch {
+   int device = 0, precision = 0, stencil = 7, cus = 256;
+   hipStream_t stream = nullptr;
+   std::vector<amr_dev *> members;
+   std::vector<char> active;
+   std::vector<int> seen;            // plan_version of each member at the last build
+
+   // work lists
+   BMember *d_mem = nullptr;
+   int *d_pre = nullptr;             // ex[3][M+1], st[M+1], rd[M+1]
+   int mem_cap = 0;
+   int M = 0, plane_max = 0;
+   int tot_ex[3] = {0, 0, 0}, tot_st = 0, tot_rd = 0;
+   bool table_ok = false;
+   int table_vpw = 0;
+   long epoch = 0;
+   std::vector<int> member_rows;     // sum_rows of each table member
+
+   // launch shape
+   int vpw = 1, st_wgs = 0, ex_wgs = 0;
+
+   // graph
+   hipGraphExec_t gexec = nullptr;
+   long gkey[8] = {0};
+
+   // timing
+   bool timing = false;
+   struct Span { hipEvent_t a, b; int cat; };
+   std::vector<Span> spans;
+   std::vector<hipEvent_t> ev_free;
+
+   char err[512] = {0};
+};
+
+namespace {
+
+int berr(amr_batch *b, int rc, const char *fmt, ...)
+{
+   if (b) {
+      va_list ap;
+      va_start(ap, fmt);
+      vsnprintf(b->err, sizeof(b->err), fmt, ap);
+      va_end(ap);
+   }
+   return rc;
+}
+
+#define BHIP(b, call)                                                                   \
+   do {                                                                                 \
+      hipError_t e_ = (call);                                                           \
+      if (e_ != hipSuccess)                                                             \
+         return berr((b), e_ == hipErrorOutOfMemory ? AMR_ERR_NOMEM : AMR_ERR_HIP,      \
+                     "%s: %s", #call, hipGetErrorString(e_));                           \
+   } while (0)
+
+hipEvent_t bevent(amr_batch *b)
+{
+   if (!b->ev_free.empty()) {
+      hipEvent_t e = b->ev_free.back();
+      b->ev_free.pop_back();
+      return e;
+   }
+   hipEvent_t e = nullptr;
+   return hipEventCreate(&e) == hipSuccess ? e : nullptr;
+}
+
+void span_begin(amr_batch *b, int cat)
+{
+   if (!b->timing)
+      return;
+   amr_batch::Span s{bevent(b), nullptr, cat};
+   if (s.a)
+      hipEventRecord(s.a, b->stream);
+   b->spans.push_back(s);
+}
+
+void span_end(amr_batch *b)
+{
+   if (!b->timing || b->spans.empty())
+      return;
+   amr_batch::Span &s = b->spans.back();
+   s.b = bevent(b);
+   if (s.b)
+      hipEventRecord(s.b, b->stream);
+}
+
+int ceil_div(int a, int b) { return (a + b - 1) / b; }
+
+// Rebuild the member table and prefix lists when a member's plan changed,
+// a member joined or left, or the stencil's variables per item changed.
+int ensure_table(amr_batch *b)
+{
+   bool need = !b->table_ok || b->table_vpw != b->vpw;
+   for (size_t i = 0; i < b->members.size() && !need; i++)
+      if (b->seen[i] != b->members[i]->plan_version)
+         need = true;
+   if (!need)
+      return AMR_OK;
+
+   std::vector<int> order;
+   for (size_t i = 0; i < b->members.size(); i++)
+      if (b->active[i])
+         order.push_back((int) i);
+   std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+      return shape_class(b->members[x]->g) > shape_class(b->members[y]->g);
+   });
+   const int M = (int) order.size();
+   std::vector<BMember> hm(M);
+   std::vector<int> pre(5 * (M + 1), 0);
+   int *ex[3] = {&pre[0], &pre[M + 1], &pre[2 * (M + 1)]};
+   int *st = &pre[3 * (M + 1)], *rd = &pre[4 * (M + 1)];
+   b->plane_max = 0;
+   b->member_rows.assign(M, 0);
+   for (int q = 0; q < M; q++) {
+      amr_dev *d = b->members[order[q]];
+      BMember &m = hm[q];
+      m.pool = d->pool;
+      m.jobs = d->d_jobs;
+      m.sorted = d->d_sorted;
+      m.partials = d->d_partials;
+      m.sums = d->d_sums;
+      m.L = d->g;
+      m.n_active = d->n_active;
+      m.num_vars = d->c.num_vars;
+      m.part_stride = d->c.max_blocks;
+      m.cls = shape_class(d->g);
+      int off = 0;
+      for (int dir = 0; dir < 3; dir++) {
+         m.count[dir] = d->count[dir];
+         m.joff[dir] = off;
+         off += d->count[dir];
+         ex[dir][q + 1] = ex[dir][q] + d->count[dir] * ceil_div(m.num_vars, kVars);
+      }
+      st[q + 1] = st[q] + d->n_active * ceil_div(m.num_vars, b->vpw);
+      rd[q + 1] = rd[q] + m.num_vars;
+      b->plane_max = std::max(b->plane_max, d->g.gy * d->g.gz);
+      b->member_rows[q] = d->d_sums ? d->sum_rows : 0;
+   }
+   BHIP(b, hipStreamSynchronize(b->stream));   // queued kernels may read the table
+   if (M > b->mem_cap) {
+      const int cap = std::max(M, 2 * b->mem_cap);
+      hipFree(b->d_mem);
+      hipFree(b->d_pre);
+      b->d_mem = nullptr;
+      b->d_pre = nullptr;
+      b->mem_cap = 0;
+      BHIP(b, hipMalloc(&b->d_mem, sizeof(BMember) * cap));
+      BHIP(b, hipMalloc(&b->d_pre, sizeof(int) * 5 * (cap + 1)));
+      b->mem_cap = cap;
+   }
+   if (!b->d_pre)
+      BHIP(b, hipMalloc(&b->d_pre, sizeof(int) * 5));
+   if (M)
+      BHIP(b, hipMemcpy(b->d_mem, hm.data(), sizeof(BMember) * M, hipMemcpyHostToDevice));
+   BHIP(b, hipMemcpy(b->d_pre, pre.data(), sizeof(int) * pre.size(), hipMemcpyHostToDevice));
+   b->M = M;
+   for (int dir = 0; dir < 3; dir++)
+      b->tot_ex[dir] = ex[dir][M];
+   b->tot_st = st[M];
+   b->tot_rd = rd[M];
+   for (size_t i = 0; i < b->members.size(); i++)
+      b->seen[i] = b->members[i]->plan_version;
+   b->table_ok = true;
+   b->table_vpw = b->vpw;
+   b->epoch++;
+   return AMR_OK;
+}
+
+int grid_of(int wgs, int total)
+{
+   const int need = ceil_div(total, kWavesPerWg);
+   return wgs > 0 ? std::min(wgs, need) : need;
+}
+
+template <typename T, int ST, int VPW>
+int launch_stencil_t(amr_batch *b, int want_partials)
+{
+   typedef typename Acc<T>::type A;
+   const int ring = 3 * VPW * b->plane_max;
+   const size_t lds = sizeof(A) * (size_t) ring * kWavesPerWg;
+   hipLaunchKernelGGL((kb_stencil<T, ST, VPW>), dim3(grid_of(b->st_wgs, b->tot_st)), dim3(256),
+                      lds, b->stream, b->d_mem, b->d_pre + 3 * (b->M + 1), b->M, want_partials,
+                      ring);
+   BHIP(b, hipGetLastError());
+   return AMR_OK;
+}
+
+template <typename T>
+int enqueue_stage_t(amr_batch *b, int stage, int permute, int row)
+{
+   static const int perm[6][3] = { {0, 1, 2}, {1, 2, 0}, {2, 0, 1},
+                                   {0, 2, 1}, {1, 0, 2}, {2, 1, 0} };
+   span_begin(b, 0);
+   for (int o = 0; o < 3; o++) {
+      const int dir = permute ? perm[stage % 6][o] : o;
+      if (!b->tot_ex[dir])
+         continue;
+      hipLaunchKernelGGL(kb_exchange<T>, dim3(grid_of(b->ex_wgs, b->tot_ex[dir])), dim3(256), 0,
+                         b->stream, b->d_mem, b->d_pre + dir * (b->M + 1), b->M, dir);
+      BHIP(b, hipGetLastError());
+   }
+   span_end(b);
+   if (b->tot_st) {
+      span_begin(b, 1);
+      const int wp = row >= 0;
+      int rc;
+      if (b->stencil == 7)
+         rc = b->vpw == 2 ? launch_stencil_t<T, 7, 2>(b, wp) : launch_stencil_t<T, 7, 1>(b, wp);
+      else
+         rc = b->vpw == 2 ? launch_stencil_t<T, 27, 2>(b, wp) : launch_stencil_t<T, 27, 1>(b, wp);
+      span_end(b);
+      if (rc)
+         return rc;
+   }
+   if (row >= 0 && b->tot_rd) {
+      span_begin(b, 2);
+      hipLaunchKernelGGL(kb_reduce, dim3(grid_of(0, b->tot_rd)), dim3(256), 0, b->stream,
+                         b->d_mem, b->d_pre + 4 * (b->M + 1), b->M, row);
+      span_end(b);
+      BHIP(b, hipGetLastError());
+   }
+   return AMR_OK;
+}
+
+int enqueue_stage(amr_batch *b, int stage, int permute, int row)
+{
+   switch (b->precision) {
+   case AMR_PREC_F64: return enqueue_stage_t<double>(b, stage, permute, row);
+   case AMR_PREC_F32: return enqueue_stage_t<float>(b, stage, permute, row);
+   default: return enqueue_stage_t<uint16_t>(b, stage, permute, row);
+   }
+}
+
+int check_row(amr_batch *b, int row)
+{
+   if (row < 0)
+      return AMR_OK;
+   for (int q = 0; q < b->M; q++)
+      if (row >= b->member_rows[q])
+         return berr(b, AMR_ERR_ARG, "checksum row %d not reserved by every member", row);
+   return AMR_OK;
+}
+
+void drop_graph(amr_batch *b)
+{
+   if (b->gexec) {
+      hipGraphExecDestroy(b->gexec);
+      b->gexec = nullptr;
+   }
+}
+
+// Time `reps` stages (no checksums) of the current shape, after one warm-up.
+int time_stages(amr_batch *b, int reps, double *ms)
+{
+   int rc = ensure_table(b);
+   if (rc)
+      return rc;
+   const bool tm = b->timing;
+   b->timing = false;
+   hipEvent_t e0 = bevent(b), e1 = bevent(b);
+   if (!e0 || !e1)
+      return berr(b, AMR_ERR_HIP, "autotune: no events");
+   rc = enqueue_stage(b, 0, 0, -1);
+   hipEventRecord(e0, b->stream);
+   for (int r = 0; r < reps && !rc; r++)
+      rc = enqueue_stage(b, r, 0, -1);
+   hipEventRecord(e1, b->stream);
+   b->timing = tm;
+   BHIP(b, hipEventSynchronize(e1));
+   float t = 0.0f;
+   BHIP(b, hipEventElapsedTime(&t, e0, e1));
+   b->ev_free.push_back(e0);
+   b->ev_free.push_back(e1);
+   *ms = t / reps;
+   return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int amr_batch_create(int device, int precision, int stencil, amr_batch **out)
+{
+   if (!out)
+      return AMR_ERR_ARG;
+   amr_batch *b = new amr_batch();
+   *out = b;
+   b->device = device;
+   b->precision = precision;
+   b->stencil = stencil;
+   if (precision < AMR_PREC_F64 || precision > AMR_PREC_BF16)
+      return berr(b, AMR_ERR_ARG, "unknown precision %d", precision);
+   if (stencil != 7 && stencil != 27)
+      return berr(b, AMR_ERR_ARG, "stencil must be 7 or 27");
+   int ndev = 0;
+   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+      return berr(b, AMR_ERR_NODEV, "no HIP device available");
+   if (device < 0 || device >= ndev)
+      return berr(b, AMR_ERR_NODEV, "device %d out of range (%d devices)", device, ndev);
+   BHIP(b, hipSetDevice(device));
+   hipDeviceProp_t prop;
+   BHIP(b, hipGetDeviceProperties(&prop, device));
+   if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+      return berr(b, AMR_ERR_NODEV, "device %d is %s, this library is built for gfx950", device,
+                  prop.gcnArchName);
+   b->cus = prop.multiProcessorCount;
+   b->vpw = precision == AMR_PREC_F64 ? 1 : 2;
+   BHIP(b, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+   return AMR_OK;
+}
+
+int amr_batch_destroy(amr_batch *b)
+{
+   if (!b)
+      return AMR_OK;
+   if (b->stream)
+      hipStreamSynchronize(b->stream);
+   drop_graph(b);
+   for (amr_dev *d : b->members)
+      amr_dev_destroy(d);
+   for (auto &s : b->spans) {
+      if (s.a) hipEventDestroy(s.a);
+      if (s.b) hipEventDestroy(s.b);
+   }
+   for (auto e : b->ev_free)
+      hipEventDestroy(e);
+   hipFree(b->d_mem);
+   hipFree(b->d_pre);
+   if (b->stream)
+      hipStreamDestroy(b->stream);
+   delete b;
+   return AMR_OK;
+}
+
+const char *amr_batch_last_error(const amr_batch *b)
+{
+   return b ? b->err : "no batch";
+}
+
+int amr_batch_add(amr_batch *b, const amr_dev_config *cfg, amr_dev **member)
+{
+   if (!b || !cfg || !member)
+      return berr(b, AMR_ERR_ARG, "add: bad arguments");
+   *member = nullptr;
+   if (!b->stream)
+      return berr(b, AMR_ERR_STATE, "add: batch not created");
+   if (cfg->precision != b->precision || cfg->stencil != b->stencil || cfg->device != b->device)
+      return berr(b, AMR_ERR_ARG, "add: member precision/stencil/device differ from the batch");
+   amr_dev *d = nullptr;
+   int rc = amr_dev_create(cfg, &d);
+   if (rc) {
+      berr(b, rc, "add: %s", d ? amr_dev_last_error(d) : "context creation failed");
+      amr_dev_destroy(d);
+      return rc;
+   }
+   if (shape_class(d->g) < 0) {
+      amr_dev_destroy(d);
+      return berr(b, AMR_ERR_ARG, "add: block face %dx%d larger than 16x16", cfg->ny, cfg->nz);
+   }
+   // members work on the batch's stream, so member calls and batch stages
+   // are ordered without events
+   hipStreamSynchronize(d->stream);
+   hipStreamDestroy(d->stream);
+   d->stream = b->stream;
+   d->owns_stream = false;
+   b->members.push_back(d);
+   b->active.push_back(1);
+   b->seen.push_back(-1);
+   b->table_ok = false;
+   *member = d;
+   return AMR_OK;
+}
+
+int amr_batch_set_active(amr_batch *b, amr_dev *member, int on)
+{
+   if (!b)
+      return AMR_ERR_ARG;
+   for (size_t i = 0; i < b->members.size(); i++)
+      if (b->members[i] == member) {
+         if ((b->active[i] != 0) != (on != 0)) {
+            b->active[i] = on != 0;
+            b->table_ok = false;
+         }
+         return AMR_OK;
+      }
+   return berr(b, AMR_ERR_ARG, "set_active: not a member of this batch");
+}
+
+int amr_batch_set_shape(amr_batch *b, int vpw, int stencil_wgs, int exchange_wgs)
+{
+   if (!b || (vpw != 1 && vpw != 2) || stencil_wgs < 0 || exchange_wgs < 0)
+      return berr(b, AMR_ERR_ARG, "set_shape: vpw must be 1 or 2, grid sizes >= 0");
+   b->vpw = vpw;
+   b->st_wgs = stencil_wgs;
+   b->ex_wgs = exchange_wgs;
+   return AMR_OK;
+}
+
+int amr_batch_get_shape(const amr_batch *b, int shape[3])
+{
+   if (!b || !shape)
+      return AMR_ERR_ARG;
+   shape[0] = b->vpw;
+   shape[1] = b->st_wgs;
+   shape[2] = b->ex_wgs;
+   return AMR_OK;
+}
+
+int amr_batch_stage(amr_batch *b, int stage, int permute, int sum_row)
+{
+   if (!b || stage < 0)
+      return berr(b, AMR_ERR_ARG, "stage: bad arguments");
+   int rc = ensure_table(b);
+   if (!rc)
+      rc = check_row(b, sum_row);
+   if (!rc)
+      rc = enqueue_stage(b, stage, permute, sum_row);
+   return rc;
+}
+
+int amr_batch_timestep(amr_batch *b, int stage0, int nstages, int checksum_freq, int permute)
+{
+   if (!b || stage0 < 0 || nstages < 0 || checksum_freq < 0)
+      return berr(b, AMR_ERR_ARG, "timestep: bad arguments");
+   if (!nstages)
+      return AMR_OK;
+   int rc = ensure_table(b);
+   if (rc)
+      return rc;
+   if (checksum_freq && (rc = check_row(b, (nstages - 1) / checksum_freq)))
+      return rc;
+   const long key[8] = {b->epoch, permute ? stage0 % 6 : -1, nstages, checksum_freq,
+                        b->vpw, b->st_wgs, b->ex_wgs, (long) b->stencil};
+   if (!b->gexec || memcmp(key, b->gkey, sizeof(key))) {
+      drop_graph(b);
+      const bool tm = b->timing;
+      b->timing = false;   // no event records inside the graph
+      BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      for (int s = 0; s < nstages && !rc; s++) {
+         const int row = (checksum_freq && !(s % checksum_freq)) ? s / checksum_freq : -1;
+         rc = enqueue_stage(b, stage0 + s, permute, row);
+      }
+      hipGraph_t graph = nullptr;
+      hipError_t e = hipStreamEndCapture(b->stream, &graph);
+      b->timing = tm;
+      if (rc) {
+         if (graph) hipGraphDestroy(graph);
+         return rc;
+      }
+      if (e != hipSuccess)
+         return berr(b, AMR_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+      e = hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0);
+      hipGraphDestroy(graph);
+      if (e != hipSuccess) {
+         b->gexec = nullptr;
+         return berr(b, AMR_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+      }
+      memcpy(b->gkey, key, sizeof(key));
+   }
+   span_begin(b, 3);
+   BHIP(b, hipGraphLaunch(b->gexec, b->stream));
+   span_end(b);
+   return AMR_OK;
+}
+
+int amr_batch_autotune(amr_batch *b, double *ms_out)
+{
+   if (!b)
+      return AMR_ERR_ARG;
+   int rc = ensure_table(b);
+   if (rc)
+      return rc;
+   if (!b->tot_st)
+      return AMR_OK;
+   // snapshot the pools of the active members
+   std::vector<std::pair<amr_dev *, size_t>> pools;
+   size_t total = 0;
+   for (size_t i = 0; i < b->members.size(); i++)
+      if (b->active[i]) {
+         amr_dev *d = b->members[i];
+         const size_t bytes = (size_t) d->g.var_stride * d->c.num_vars * d->elem_bytes;
+         pools.push_back({d, bytes});
+         total += bytes;
+      }
+   char *backup = nullptr;
+   BHIP(b, hipMalloc(&backup, total));
+   size_t off = 0;
+   for (auto &p : pools) {
+      BHIP(b, hipMemcpyAsync(backup + off, p.first->pool, p.second, hipMemcpyDeviceToDevice,
+                             b->stream));
+      off += p.second;
+   }
+   const int reps = 3;
+   const int vpws[2] = {1, 2};
+   const int mult[5] = {0, 1, 2, 4, 8};
+   double best = 1e30;
+   int bv = b->vpw, bs = b->st_wgs, be = b->ex_wgs;
+   // stencil shape with the exchange fixed, then the exchange grid
+   for (int vi = 0; vi < 2 && !rc; vi++)
+      for (int si = 0; si < 5 && !rc; si++) {
+         b->vpw = vpws[vi];
+         b->st_wgs = mult[si] * b->cus;
+         b->ex_wgs = be;
+         double t = 0.0;
+         rc = time_stages(b, reps, &t);
+         if (!rc && t < best) {
+            best = t;
+            bv = b->vpw;
+            bs = b->st_wgs;
+         }
+      }
+   for (int ei = 0; ei < 5 && !rc; ei++) {
+      b->vpw = bv;
+      b->st_wgs = bs;
+      b->ex_wgs = mult[ei] * b->cus;
+      double t = 0.0;
+      rc = time_stages(b, reps, &t);
+      if (!rc && t < best) {
+         best = t;
+         be = b->ex_wgs;
+      }
+   }
+   b->vpw = bv;
+   b->st_wgs = bs;
+   b->ex_wgs = be;
+   off = 0;
+   for (auto &p : pools) {
+      hipMemcpyAsync(p.first->pool, backup + off, p.second, hipMemcpyDeviceToDevice, b->stream);
+      off += p.second;
+   }
+   hipError_t e = hipStreamSynchronize(b->stream);
+   hipFree(backup);
+   drop_graph(b);
+   if (rc)
+      return rc;
+   if (e != hipSuccess)
+      return berr(b, AMR_ERR_HIP, "autotune restore: %s", hipGetErrorString(e));
+   if (ms_out)
+      *ms_out = best;
+   return ensure_table(b);
+}
+
+int amr_batch_sync(amr_batch *b)
+{
+   if (!b)
+      return AMR_ERR_ARG;
+   BHIP(b, hipStreamSynchronize(b->stream));
+   return AMR_OK;
+}
+
+int amr_batch_set_timing(amr_batch *b, int on)
+{
+   if (!b)
+      return AMR_ERR_ARG;
+   b->timing = on != 0;
+   return AMR_OK;
+}
+
+int amr_batch_read_timing(amr_batch *b, double ms[4])
+{
+   if (!b || !ms)
+      return AMR_ERR_ARG;
+   for (int i = 0; i < 4; i++)
+      ms[i] = 0.0;
+   BHIP(b, hipStreamSynchronize(b->stream));
+   for (auto &s : b->spans) {
+      float t = 0.0f;
+      if (s.a && s.b && hipEventElapsedTime(&t, s.a, s.b) == hipSuccess)
+         ms[s.cat] += t;
+      if (s.a) b->ev_free.push_back(s.a);
+      if (s.b) b->ev_free.push_back(s.b);
+   }
+   b->spans.clear();
+   return AMR_OK;
+}
+
+}  // extern "C"

@@ -152,7 +152,7 @@ int amr_dev_destroy(amr_dev *d)
    hipFree(d->d_jobbuf);
    if (d->h_pinned)
       hipHostFree(d->h_pinned);
-   if (d->stream)
+   if (d->stream && d->owns_stream)
       hipStreamDestroy(d->stream);
    delete d;
    return AMR_OK;
@@ -231,6 +231,7 @@ int amr_dev_set_mesh(amr_dev *d, const int32_t *sorted, int n_active,
    d->n_active = n_active;
    for (int i = 0; i < 3; i++)
       d->count[i] = count[i];
+   d->plan_version++;
    return AMR_OK;
 }
 
@@ -379,6 +380,7 @@ int amr_dev_reserve_sums(amr_dev *d, int n_rows)
    HIP_TRY(d, hipMalloc(&d->d_sums, sizeof(double) * (size_t) n_rows * d->c.num_vars));
    HIP_TRY(d, hipMemset(d->d_sums, 0, sizeof(double) * (size_t) n_rows * d->c.num_vars));
    d->sum_rows = n_rows;
+   d->plan_version++;
    size_t need = sizeof(double) * (size_t) n_rows * d->c.num_vars;
    if (need > d->h_pinned_bytes) {
       hipHostFree(d->h_pinned);

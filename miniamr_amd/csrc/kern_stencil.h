@@ -1,0 +1,222 @@
+// kern_stencil.h -- device body of the 7/27-point stencil on one block
+// record, shared by the single-instance kernel (stencil.hip) and the batch
+// persistent kernel (batch.hip).  See stencil.hip for the algorithm.
+#pragma once
+
+#include "amr_internal.h"
+
+namespace amrk {
+
+constexpr int kThreads = 256;
+constexpr int kCols = 4;       // columns per thread: ny*nz <= 1024
+constexpr int kPer = 8;        // plane elements per thread: gy*gz <= 2048
+constexpr int kDepth = 4;      // planes of loads in flight per workgroup
+
+// Synchronisation and reduction over the NT threads that share one work
+// item: the whole workgroup (NT == kThreads) or one wavefront (NT == 64).
+// A wavefront's LDS operations complete in issue order, so within a wave
+// a compiler barrier between the ring writes and reads is enough.
+template <int NT>
+__device__ __forceinline__ void item_sync()
+{
+   if constexpr (NT == 64) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+   } else {
+      __syncthreads();
+   }
+}
+
+__device__ __forceinline__ double wave_reduce(double v)
+{
+   for (int o = 32; o > 0; o >>= 1)
+      v += __shfl_down(v, o, 64);
+   return v;   // valid in lane 0
+}
+
+__device__ __forceinline__ double block_reduce(double v, double *red)
+{
+   for (int o = 32; o > 0; o >>= 1)
+      v += __shfl_down(v, o, 64);
+   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+   __syncthreads();
+   if (lane == 0)
+      red[wave] = v;
+   __syncthreads();
+   double s = 0.0;
+   if (threadIdx.x == 0)
+      for (int w = 0; w < (int) (blockDim.x >> 6); w++)
+         s += red[w];
+   return s;   // valid in thread 0
+}
+
+template <typename A>
+__device__ __forceinline__ A sum9(const A *pl, int c, int gz)
+{
+   A a = pl[c - gz - 1] + pl[c - gz];
+   a = a + pl[c - gz + 1];
+   a = a + pl[c - 1];
+   a = a + pl[c];
+   a = a + pl[c + 1];
+   a = a + pl[c + gz - 1];
+   a = a + pl[c + gz];
+   a = a + pl[c + gz + 1];
+   return a;
+}
+
+// Division of the 27- and 7-point sums.  FP64: IEEE division, so the path
+// is bit-identical to the reference.  FP32/BF16 forms: division by a known
+// constant with one FMA correction of the reciprocal product (Markstein's
+// method for a fixed divisor): q = x*r, e = fma(-q, d, x), q' = fma(e, r, q)
+// with r = RN(1/d).  Unlike a bare x*r it carries no systematic bias (RN(1/7)
+// alone is 4.5e-8 high, which would drift conserved sums by ~1e-5 over 400
+// stages); on 2e6 random operands it agreed with IEEE division everywhere.
+template <typename A, int DIV>
+__device__ __forceinline__ A divide(A w)
+{
+   if (sizeof(A) == 8)
+      return w / A(DIV);
+   const float r = 1.0f / (float) DIV;
+   const float q = (float) w * r;
+   const float e = __fmaf_rn(-q, (float) DIV, (float) w);
+   return (A) __fmaf_rn(e, r, q);
+}
+
+// One (block, up to VPW variables) item, computed by NT threads: the whole
+// workgroup (NT == kThreads) or one wavefront (NT == 64).  rec points at
+// the block's record of the first variable; `ring` is 3*VPW planes of LDS
+// owned by the item; `red` 4 doubles of LDS (workgroup items only);
+// `partials`, when not null, receives the FP64 sum of the stored values of
+// variable u at partials[u * part_stride].  Requires ny*nz <= NCOL*NT and
+// (ny+2)*(nz+2) <= NPER*NT.
+template <typename T, int ST, int NCOL, int NPER, int VPW, int NT = kThreads>
+__device__ __forceinline__ void stencil_block(T *__restrict__ rec, const Lay &L, int nv,
+                                              typename Acc<T>::type *ring, double *red,
+                                              double *__restrict__ partials, int part_stride)
+{
+   typedef typename Acc<T>::type A;
+   constexpr int D = kDepth;           // planes of global loads in flight
+   const int plane = L.gy * L.gz, ncol = L.ny * L.nz, nplanes = L.nx + 2;
+   const int64_t vs = L.var_stride;
+   const int tid = NT == kThreads ? (int) threadIdx.x : (int) (threadIdx.x & (NT - 1));
+
+   // Record offsets of this thread's plane elements: plane 0, plane nx+1,
+   // and interior planes p, where the offset is affine in p.
+   int off0[NPER], offN[NPER], off1[NPER], stp[NPER];
+#pragma unroll
+   for (int q = 0; q < NPER; q++) {
+      const int e = min(tid + q * NT, plane - 1);
+      const int jj = e / L.gz, kk = e - jj * L.gz;
+      off0[q] = home(L, 0, jj, kk);
+      offN[q] = home(L, L.nx + 1, jj, kk);
+      off1[q] = home(L, 1, jj, kk);
+      stp[q] = L.nx > 1 ? home(L, 2, jj, kk) - off1[q] : 0;
+   }
+   T pre[D][VPW][NPER];
+#define PREFETCH(slot, p)                                                  \
+   _Pragma("unroll") for (int q = 0; q < NPER; q++) {                      \
+      if (tid + q * NT < plane) {                                          \
+         const int o = (p) == 0 ? off0[q] : (p) == L.nx + 1 ? offN[q]       \
+                                         : off1[q] + ((p) - 1) * stp[q];    \
+         _Pragma("unroll") for (int u = 0; u < VPW; u++)                   \
+            if (u < nv) pre[slot][u][q] = rec[u * vs + o];                 \
+      }                                                                    \
+   }
+
+   A s_prev[VPW][NCOL], s_cur[VPW][NCOL], c_prev[VPW][NCOL];
+   double bsum[VPW];
+#pragma unroll
+   for (int u = 0; u < VPW; u++)
+      bsum[u] = 0.0;
+#pragma unroll
+   for (int d = 0; d < D; d++)
+      if (d < nplanes) { PREFETCH(d, d); }
+
+   for (int p0 = 0; p0 < nplanes; p0 += D) {
+#pragma unroll
+      for (int d = 0; d < D; d++) {
+         const int p = p0 + d;
+         if (p >= nplanes)
+            break;
+         // park plane p (ring slot p % 3 was last read two planes ago)
+#pragma unroll
+         for (int u = 0; u < VPW; u++) {
+            A *slot = ring + (u * 3 + p % 3) * plane;
+#pragma unroll
+            for (int q = 0; q < NPER; q++) {
+               const int e = tid + q * NT;
+               if (e < plane && u < nv)
+                  slot[e] = to_acc(pre[d][u][q]);
+            }
+         }
+         if (p + D < nplanes) { PREFETCH(d, p + D); }
+         item_sync<NT>();
+#pragma unroll
+         for (int u = 0; u < VPW; u++) {
+            if (u >= nv)
+               continue;
+            const A *slot = ring + (u * 3 + p % 3) * plane;
+            const A *pc = ring + (u * 3 + (p + 2) % 3) * plane;   // plane p - 1
+            T *ru = rec + u * vs;
+#pragma unroll
+            for (int q = 0; q < NCOL; q++) {
+               const int col = tid + q * NT;
+               if (col >= ncol)
+                  continue;
+               const int j = col / L.nz + 1, k = col - (j - 1) * L.nz + 1, c = j * L.gz + k;
+               A w = 0;
+               bool out_now = p >= 2;
+               if (ST == 27) {
+                  const A s_new = sum9(slot, c, L.gz);
+                  if (out_now) {
+                     w = (s_prev[u][q] + s_cur[u][q]) + s_new;
+                     w = divide<A, 27>(w);
+                  }
+                  s_prev[u][q] = s_cur[u][q];
+                  s_cur[u][q] = s_new;
+               } else {
+                  if (out_now) {
+                     const A cc = pc[c];
+                     w = c_prev[u][q] + pc[c - L.gz];
+                     w = w + pc[c - 1];
+                     w = w + cc;
+                     w = w + pc[c + 1];
+                     w = w + pc[c + L.gz];
+                     w = w + slot[c];
+                     w = divide<A, 7>(w);
+                     c_prev[u][q] = cc;
+                  } else if (p == 0) {
+                     c_prev[u][q] = slot[c];
+                  }
+               }
+               if (out_now) {
+                  const T out = from_acc<T>(w);
+                  const int i = p - 1;
+                  ru[((i - 1) * L.ny + (j - 1)) * L.nz + (k - 1)] = out;
+                  if (k == 1 || k == L.nz)
+                     ru[zcopy(L, i, j, k)] = out;
+                  if (partials)
+                     bsum[u] += (double) to_acc(out);
+               }
+            }
+         }
+      }
+   }
+#undef PREFETCH
+   if (partials) {
+#pragma unroll
+      for (int u = 0; u < VPW; u++) {
+         double s;
+         if constexpr (NT == 64)
+            s = wave_reduce(bsum[u]);
+         else
+            s = block_reduce(bsum[u], red);
+         if (tid == 0 && u < nv)
+            partials[(int64_t) u * part_stride] = s;
+      }
+   }
+   item_sync<NT>();   // the ring (and red[]) are reused by the next item
+}
+
+}  // namespace amrk

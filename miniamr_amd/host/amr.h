@@ -46,6 +46,8 @@ typedef struct {
    int schedule;         /* 0: stage-major like the reference,
                             1: variable-major within a refinement interval */
    int device;
+   int seed;             /* --seed: generator seed of the initial values
+                            (1 = the reference's implicit srand(1)) */
 } amr_params;
 
 /* A block slot (ref/block.h:36-53 minus off-rank fields). */
@@ -124,6 +126,8 @@ typedef struct amr_run {
 
    /* device */
    amr_dev *dev;
+   struct amr_batch *batch;    /* when set, dev is created as a member of it */
+   FILE *sum_log;              /* checksum log (AMR_DUMP_SUMS), or NULL */
    amr_vec split_jobs, merge_jobs;
    int mesh_dirty;             /* exchange plan must be rebuilt */
    amr_vec face_jobs;
@@ -167,7 +171,13 @@ void amr_consolidate_blocks(amr_run *R);
 int amr_driver(amr_run *R);
 int amr_driver_begin(amr_run *R);
 int amr_driver_step(amr_run *R);
+int amr_driver_step_begin(amr_run *R);
+void amr_driver_step_enqueue(amr_run *R);
+void amr_driver_step_advance(amr_run *R);
+int amr_driver_step_end(amr_run *R);
 void amr_driver_end(amr_run *R);
+void amr_driver_set_sum_log(amr_run *R, const char *path);
+int amr_dump_final_to(amr_run *R, const char *cells_path, const char *bsums_path);
 double amr_calc_time_step(amr_run *R);
 void amr_report(amr_run *R);
 int amr_dump_final(amr_run *R);

@@ -35,14 +35,20 @@ double amr_checksum(amr_run *R, int var)
    return s;
 }
 
-static FILE *g_sum_log;
-
-static void log_sum(double s)
+static void log_sum(amr_run *R, double s)
 {
-   if (g_sum_log) {
-      fprintf(g_sum_log, "%.17g\n", s);
-      fflush(g_sum_log);
+   if (R->sum_log) {
+      fprintf(R->sum_log, "%.17g\n", s);
+      fflush(R->sum_log);
    }
+}
+
+/* Checksum log of this run (the AMR_DUMP_SUMS format); NULL closes it. */
+void amr_driver_set_sum_log(amr_run *R, const char *path)
+{
+   if (R->sum_log)
+      fclose(R->sum_log);
+   R->sum_log = path ? fopen(path, "w") : NULL;
 }
 
 /* ref/driver.c:171-214 */
@@ -138,7 +144,7 @@ static int check_timestep(amr_run *R, int ts, const double *sums, int64_t *block
       for (int var = 0; var < V; var++) {
          double sum = row[var];
          R->total_red++;
-         log_sum(sum);
+         log_sum(R, sum);
          if (P->report_diffusion)
             printf("%d var %d sum %lf old %lf diff %lf %lf tol %lf\n", ts, var, sum,
                    R->grid_sum[var], (sum - R->grid_sum[var]),
@@ -161,12 +167,13 @@ int amr_driver_begin(amr_run *R)
    amr_params *P = &R->P;
    amr_loop *L = &R->loop;
    const char *sp = getenv("AMR_DUMP_SUMS");
-   g_sum_log = sp ? fopen(sp, "w") : NULL;
+   if (sp && !R->sum_log && !R->batch)
+      amr_driver_set_sum_log(R, sp);
 
    double t2 = amr_now();
    amr_init(R);
    for (int v = 0; v < P->num_vars; v++)
-      log_sum(R->grid_sum[v]);
+      log_sum(R, R->grid_sum[v]);
    double t1 = amr_now();
    R->t_init = t1 - t2;
 
@@ -194,19 +201,40 @@ int amr_driver_begin(amr_run *R)
    return 0;
 }
 
-/* One timestep of ref/driver.c:67-160.  Returns 0 while more timesteps
- * follow, 1 after the last one, 2 if a conservation check stopped the run. */
-int amr_driver_step(amr_run *R)
+/* One timestep of ref/driver.c:67-160, in phases so that a batch can
+ * enqueue the stages of many runs together:
+ *   begin    -- returns the run's state (nonzero: nothing more to do),
+ *               else prints the timestep line;
+ *   enqueue  -- this run's stages on its own device context;
+ *   advance  -- after the stages were enqueued (by enqueue or a batch);
+ *   end      -- checksums, conservation test, motion and refinement.
+ * Returns 0 while more timesteps follow, 1 after the last one, 2 if a
+ * conservation check stopped the run. */
+int amr_driver_step_begin(amr_run *R)
 {
-   amr_params *P = &R->P;
    amr_loop *L = &R->loop;
    if (L->state)
       return L->state;
+   if (R->P.report_perf & 8)
+      printf("Timestep %d time %lf delta %lf\n", L->ts, L->sim_time, L->delta);
+   return 0;
+}
+
+void amr_driver_step_enqueue(amr_run *R)
+{
+   enqueue_timestep(R, R->loop.comm_stage);
+}
+
+void amr_driver_step_advance(amr_run *R)
+{
+   R->loop.comm_stage += R->P.stages_per_ts;
+}
+
+int amr_driver_step_end(amr_run *R)
+{
+   amr_params *P = &R->P;
+   amr_loop *L = &R->loop;
    int ts = L->ts;
-   if (P->report_perf & 8)
-      printf("Timestep %d time %lf delta %lf\n", ts, L->sim_time, L->delta);
-   enqueue_timestep(R, L->comm_stage);
-   L->comm_stage += P->stages_per_ts;
    if (L->n_rows)
       amr_dev_check(R, amr_dev_read_sums(R->dev, 0, L->n_rows, L->sums), "read_sums");
    else
@@ -253,6 +281,16 @@ int amr_driver_step(amr_run *R)
    return L->state;
 }
 
+int amr_driver_step(amr_run *R)
+{
+   int st = amr_driver_step_begin(R);
+   if (st)
+      return st;
+   amr_driver_step_enqueue(R);
+   amr_driver_step_advance(R);
+   return amr_driver_step_end(R);
+}
+
 /* ref/driver.c:162-164 and the timer bookkeeping for the report. */
 void amr_driver_end(amr_run *R)
 {
@@ -272,9 +310,7 @@ void amr_driver_end(amr_run *R)
    }
    free(L->sums);
    L->sums = NULL;
-   if (g_sum_log)
-      fclose(g_sum_log);
-   g_sum_log = NULL;
+   amr_driver_set_sum_log(R, NULL);
 }
 
 /* ref/driver.c:37-165 */
@@ -325,9 +361,12 @@ void amr_report(amr_run *R)
 /* Final-state dumps in the formats of the golden-vector harness. */
 int amr_dump_final(amr_run *R)
 {
+   return amr_dump_final_to(R, getenv("AMR_DUMP_CELLS"), getenv("AMR_DUMP_BSUMS"));
+}
+
+int amr_dump_final_to(amr_run *R, const char *pc, const char *pb)
+{
    const amr_params *P = &R->P;
-   const char *pc = getenv("AMR_DUMP_CELLS");
-   const char *pb = getenv("AMR_DUMP_BSUMS");
    if (!pc && !pb)
       return 0;
    int gx = P->nx + 2, gy = P->ny + 2, gz = P->nz + 2;

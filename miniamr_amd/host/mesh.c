@@ -4,7 +4,7 @@
  * Initial mesh: ref/init.c:38-683 for one rank with the RCB/Morton layout
  * (lb_method < 2): blocks numbered x-fastest, slot == number, level 0,
  * reflective faces on the domain boundary, interior values drawn from the
- * glibc generator behind rand() with its default seed, in the reference's
+ * glibc generator behind rand() with its default seed (or --seed), in the reference's
  * order (block, var, i, j, k), so the field equals the reference's.
  *
  * Sorted list: ref/block.c:610-657 -- active blocks grouped by level, each
@@ -25,6 +25,7 @@
 #include <time.h>
 
 #include "amr.h"
+#include "amr_batch.h"
 
 void amr_fatal(const char *fmt, ...)
 {
@@ -105,6 +106,13 @@ int amr_allocate(amr_run *R)
    cfg.stencil = P->stencil;
    cfg.precision = P->precision;
    cfg.device = P->device;
+   if (R->batch) {
+      int rc = amr_batch_add(R->batch, &cfg, &R->dev);
+      if (rc != AMR_OK)
+         amr_fatal("ERROR: could not add the device context to the batch (%d): %s\n", rc,
+                   amr_batch_last_error(R->batch));
+      return 0;
+   }
    int rc = amr_dev_create(&cfg, &R->dev);
    if (rc != AMR_OK)
       amr_fatal("ERROR: could not create the device context (%d): %s\n", rc,
@@ -114,7 +122,7 @@ int amr_allocate(amr_run *R)
 
 void amr_free(amr_run *R)
 {
-   if (R->dev)
+   if (R->dev && !R->batch)   /* batch members are destroyed with the batch */
       amr_dev_destroy(R->dev);
    free(R->blocks); free(R->parents); free(R->sorted_number); free(R->sorted_slot);
    free(R->sorted_index); free(R->num_blocks); free(R->block_start);
@@ -171,7 +179,7 @@ int amr_init(amr_run *R)
 
    R->tol = pow(10.0, (double) -P->error_tol);
    memset(&R->rng, 0, sizeof(R->rng));
-   initstate_r(1, R->rng_state, sizeof(R->rng_state), &R->rng);
+   initstate_r((unsigned int) P->seed, R->rng_state, sizeof(R->rng_state), &R->rng);
    R->p2[0] = 1;
    R->p8[0] = 1;
    for (int i = 0; i <= nr; i++) {

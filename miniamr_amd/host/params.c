@@ -35,6 +35,7 @@ static void set_defaults(amr_params *P)
    P->report_perf = 12;
    P->lb_opt = 1;
    P->nonblocking = 1;
+   P->seed = 1;
 }
 
 void amr_print_help(void)
@@ -72,6 +73,10 @@ void amr_print_help(void)
       "--rcb, --morton - load balance method (identical with one rank)",
       "--num_objects - (>= 0) number of objects to cause refinement",
       "--object - type, bounce, position(3), movement(3), size(3), size rate of change(3)",
+      "",
+      "Extensions of this build (not in the reference):",
+      "--seed - seed of the initial values' generator (default 1, the reference's)",
+      "--precision - storage of the cell values: f64 (default), f32 or bf16",
       "",
       "Environment: MINIAMR_PRECISION=f64|f32|bf16 storage precision (default f64),",
       "             MINIAMR_SCHEDULE=stage|var stage-major or variable-major order,",
@@ -121,6 +126,7 @@ int amr_parse_args(amr_params *P, int argc, char **argv)
       {"--checksum_freq", &P->checksum_freq, 0}, {"--stencil", &P->stencil, 0},
       {"--report_perf", &P->report_perf, 0}, {"--plot_freq", &P->plot_freq, 0},
       {"--code", &P->code, 0}, {"--limit_move", &P->limit_move, 0},
+      {"--seed", &P->seed, 0},
       {NULL, NULL, 0} };
    /* flags without operand: flag, destination, value stored */
    struct { const char *flag; int *dst; int value; } sflags[] = {
@@ -132,6 +138,7 @@ int amr_parse_args(amr_params *P, int argc, char **argv)
       {"--hilbert", &P->lb_method, 2}, {"--trunc_hilbert", &P->lb_method, 3},
       {NULL, NULL, 0} };
    int object_num = 0;
+   const char *prec_flag = NULL;
 
    set_defaults(P);
    for (int i = 1; i < argc; i++) {
@@ -172,6 +179,10 @@ int amr_parse_args(amr_params *P, int argc, char **argv)
          for (int d = 0; d < 3; d++) o->move[d] = dbl_arg(argc, argv, &i);
          for (int d = 0; d < 3; d++) o->size[d] = dbl_arg(argc, argv, &i);
          for (int d = 0; d < 3; d++) o->inc[d] = dbl_arg(argc, argv, &i);
+      } else if (!strcmp(a, "--precision")) {
+         if (i + 1 >= argc)
+            die_usage("** Error ** missing value for %s\n", a);
+         prec_flag = argv[++i];
       } else if (!strcmp(a, "--help")) {
          die_usage(NULL, NULL);
       } else {
@@ -193,14 +204,14 @@ int amr_parse_args(amr_params *P, int argc, char **argv)
 
    /* build extensions, from the environment so the command line stays the
     * reference's */
-   const char *prec = getenv("MINIAMR_PRECISION");
+   const char *prec = prec_flag ? prec_flag : getenv("MINIAMR_PRECISION");
    P->precision = AMR_PREC_F64;
    if (prec && !strcmp(prec, "f32"))
       P->precision = AMR_PREC_F32;
    else if (prec && !strcmp(prec, "bf16"))
       P->precision = AMR_PREC_BF16;
    else if (prec && strcmp(prec, "f64")) {
-      printf("MINIAMR_PRECISION must be f64, f32 or bf16\n");
+      printf("precision must be f64, f32 or bf16\n");
       exit(-1);
    }
    const char *sch = getenv("MINIAMR_SCHEDULE");

@@ -358,3 +358,134 @@ int amr_dev_read_timing(amr_dev *d, double ms[4])
    ms[0] = ms[1] = ms[2] = ms[3] = 0.0;
    return AMR_OK;
 }
+
+/* ------------------------------------------------------------------------
+ * include/amr_batch.h on the CPU: the batch's definition restated as "each
+ * member steps alone" (members in order, all variables per stage), which
+ * is what the batched kernels must reproduce.  Launch shapes and tuning
+ * have no meaning here and are accepted and ignored.
+ * ---------------------------------------------------------------------- */
+#include "amr_batch.h"
+
+struct amr_batch {
+   int device, precision, stencil;
+   amr_dev **members;
+   char *active;
+   int n, cap;
+   int shape[3];
+   char err[256];
+};
+
+int amr_batch_create(int device, int precision, int stencil, amr_batch **out)
+{
+   amr_batch *b = (amr_batch *) calloc(1, sizeof(amr_batch));
+   *out = b;
+   b->device = device;
+   b->precision = precision;
+   b->stencil = stencil;
+   b->shape[0] = 1;
+   if (precision != AMR_PREC_F64) {
+      snprintf(b->err, sizeof(b->err), "the CPU oracle is FP64 only");
+      return AMR_ERR_ARG;
+   }
+   return AMR_OK;
+}
+
+int amr_batch_destroy(amr_batch *b)
+{
+   if (!b)
+      return AMR_OK;
+   for (int i = 0; i < b->n; i++)
+      amr_dev_destroy(b->members[i]);
+   free(b->members);
+   free(b->active);
+   free(b);
+   return AMR_OK;
+}
+
+const char *amr_batch_last_error(const amr_batch *b) { return b ? b->err : "no batch"; }
+
+int amr_batch_add(amr_batch *b, const amr_dev_config *cfg, amr_dev **member)
+{
+   if (cfg->precision != b->precision || cfg->stencil != b->stencil) {
+      snprintf(b->err, sizeof(b->err), "member differs from the batch");
+      return AMR_ERR_ARG;
+   }
+   int rc = amr_dev_create(cfg, member);
+   if (rc) {
+      snprintf(b->err, sizeof(b->err), "%s", *member ? amr_dev_last_error(*member) : "create");
+      return rc;
+   }
+   if (b->n == b->cap) {
+      b->cap = b->cap ? 2 * b->cap : 16;
+      b->members = (amr_dev **) realloc(b->members, sizeof(amr_dev *) * b->cap);
+      b->active = (char *) realloc(b->active, b->cap);
+   }
+   b->members[b->n] = *member;
+   b->active[b->n++] = 1;
+   return AMR_OK;
+}
+
+int amr_batch_set_active(amr_batch *b, amr_dev *member, int on)
+{
+   for (int i = 0; i < b->n; i++)
+      if (b->members[i] == member) {
+         b->active[i] = on != 0;
+         return AMR_OK;
+      }
+   return AMR_ERR_ARG;
+}
+
+int amr_batch_set_shape(amr_batch *b, int vpw, int stencil_wgs, int exchange_wgs)
+{
+   b->shape[0] = vpw;
+   b->shape[1] = stencil_wgs;
+   b->shape[2] = exchange_wgs;
+   return AMR_OK;
+}
+
+int amr_batch_get_shape(const amr_batch *b, int shape[3])
+{
+   memcpy(shape, b->shape, sizeof(b->shape));
+   return AMR_OK;
+}
+
+int amr_batch_stage(amr_batch *b, int stage, int permute, int sum_row)
+{
+   for (int i = 0; i < b->n; i++)
+      if (b->active[i]) {
+         amr_dev *d = b->members[i];
+         int rc = amr_dev_stage(d, 0, d->c.num_vars, stage, permute, sum_row);
+         if (rc)
+            return rc;
+      }
+   return AMR_OK;
+}
+
+int amr_batch_timestep(amr_batch *b, int stage0, int nstages, int checksum_freq, int permute)
+{
+   for (int s = 0; s < nstages; s++) {
+      int row = (checksum_freq && !(s % checksum_freq)) ? s / checksum_freq : -1;
+      int rc = amr_batch_stage(b, stage0 + s, permute, row);
+      if (rc)
+         return rc;
+   }
+   return AMR_OK;
+}
+
+int amr_batch_autotune(amr_batch *b, double *ms)
+{
+   (void) b;
+   if (ms)
+      *ms = 0.0;
+   return AMR_OK;
+}
+
+int amr_batch_sync(amr_batch *b) { (void) b; return AMR_OK; }
+int amr_batch_set_timing(amr_batch *b, int on) { (void) b; (void) on; return AMR_OK; }
+int amr_batch_read_timing(amr_batch *b, double ms[4])
+{
+   (void) b;
+   ms[0] = ms[1] = ms[2] = ms[3] = 0.0;
+   return AMR_OK;
+}

@@ -19,7 +19,9 @@ REPO = os.path.dirname(HERE)
 GOLDEN = os.path.join(HERE, "golden")
 ORACLE_X = os.path.join(REPO, "oracle", "_build", "miniAMR_oracle.x")
 ORACLE_SO = os.path.join(REPO, "oracle", "_build", "liboracle_dev.so")
+ORACLE_BATCH_X = os.path.join(REPO, "oracle", "_build", "miniAMR_oracle_batch.x")
 PRODUCT_X = os.path.join(REPO, "miniamr_amd", "bin", "miniAMR.x")
+PRODUCT_BATCH_X = os.path.join(REPO, "miniamr_amd", "bin", "miniAMR_batch.x")
 PRODUCT_SO = os.path.join(REPO, "miniamr_amd", "lib", "libminiamr_dev.so")
 
 SUMMARY_RE = re.compile(r"^Summary: ranks (\d+) ts (\d+) time \S+ calc \S+ comm \S+ red \S+ "
@@ -96,6 +98,70 @@ def run(exe, args, dump=None, env=None, timeout=600):
         if dump in ("bsums", "bsums_only") and os.path.exists(e["AMR_DUMP_BSUMS"]):
             rec["bsums"] = read_dump(e["AMR_DUMP_BSUMS"])
         return rec
+
+
+BATCH_RUN_RE = re.compile(r"^run (\d+): ")
+
+
+def run_batch(exe, runs, dump=None, env=None, timeout=600, flags=()):
+    """Run a batch executable on the command lines `runs` (one string per
+    run); dump is None or "cells".  Returns {"rc", "stdout", "runs"} where
+    runs[i] has the keys of run()'s record for run i."""
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "runs.txt")
+        with open(path, "w") as f:
+            f.write("\n".join(runs) + "\n")
+        e = dict(os.environ)
+        if env:
+            e.update(env)
+        e["AMR_DUMP_SUMS"] = os.path.join(td, "sums")
+        if dump == "cells":
+            e["AMR_DUMP_CELLS"] = os.path.join(td, "cells")
+        p = subprocess.run([exe] + list(flags) + [path], cwd=td, env=e, stdout=subprocess.PIPE,
+                           stderr=subprocess.STDOUT, timeout=timeout)
+        out = p.stdout.decode(errors="replace")
+        recs = []
+        for i in range(len(runs)):
+            rec = {"rc": p.returncode, "stdout": out, "sums": [], "summary": None}
+            sp = "%s.%d" % (e["AMR_DUMP_SUMS"], i)
+            if os.path.exists(sp):
+                with open(sp) as f:
+                    rec["sums"] = [float(x) for x in f.read().split()]
+            cp = "%s.%d" % (e.get("AMR_DUMP_CELLS", ""), i)
+            if dump == "cells" and os.path.exists(cp):
+                rec["cells"] = read_dump(cp)
+            recs.append(rec)
+        cur = None   # "run i: " opens run i's report; its Summary line follows
+        for line in out.splitlines():
+            m = BATCH_RUN_RE.match(line)
+            if m:
+                cur = int(m.group(1))
+                line = line[m.end():]
+            m = SUMMARY_RE.match(line)
+            if m and cur is not None and cur < len(recs):
+                recs[cur]["summary"] = {
+                    "ranks": int(m.group(1)), "ts": int(m.group(2)),
+                    "blocks_per_ts": float(m.group(3)), "max_blocks": int(m.group(4))}
+        return {"rc": p.returncode, "stdout": out, "runs": recs}
+
+
+def same_run(a, b, sum_rtol=0.0, max_ulp=0):
+    """Assert that two runs of the same command line agree: summary, number
+    of checksums, checksums within sum_rtol, slots, and every cell within
+    max_ulp (both sides written by this project's code, so every cell is
+    defined)."""
+    assert a["summary"] is not None and a["summary"] == b["summary"], (a["summary"], b["summary"])
+    assert len(a["sums"]) == len(b["sums"]) > 0
+    e = rel_err(a["sums"], b["sums"])
+    assert e.max() <= sum_rtol, "checksum rel err %.3e > %.1e" % (e.max(), sum_rtol)
+    report = {"n_sums": len(a["sums"]), "max_sum_rel": float(e.max())}
+    if "cells" in a or "cells" in b:
+        ca, cb = a["cells"], b["cells"]
+        assert np.array_equal(ca["number"], cb["number"]) and np.array_equal(ca["slot"], cb["slot"])
+        u = ulp_diff(ca["cells"], cb["cells"])
+        report["max_cell_ulp"] = int(u.max()) if u.size else 0
+        assert report["max_cell_ulp"] <= max_ulp, "cell ulp diff %d" % report["max_cell_ulp"]
+    return report
 
 
 def rel_err(a, b):

@@ -1,5 +1,5 @@
-"""The C ABI boundary (CPU): the built libraries load and export exactly the
-entry points include/amr_dev.h declares.  No compute call is made here."""
+"""The C ABI boundary (CPU): the built libraries load and export every entry
+point include/*.h declares.  No compute call is made here."""
 import ctypes
 import os
 import re
@@ -12,10 +12,36 @@ import miniamr_amd as M
 from conftest import require
 
 
-def header_symbols():
-    with open(M.INCLUDE) as f:
+def header_symbols(header=M.INCLUDE, prefix="amr_dev_"):
+    with open(header) as f:
         text = f.read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(amr_dev_\w+)\s*\(", text, re.M)))
+    decl = r"^\s*(?:int|void|double|const char \*|amr_\w+ \*)\s*(%s\w+)\s*\(" % prefix
+    return sorted(set(re.findall(decl, text, re.M)))
+
+
+INC = os.path.dirname(M.INCLUDE)
+HOST_SO = os.path.join(M.PKG, "lib", "libminiamr_host.so")
+
+
+def device_symbols():
+    """Entry points of the device library: amr_dev.h and the device half of
+    amr_batch.h."""
+    batch = [s for s in header_symbols(os.path.join(INC, "amr_batch.h"), "amr_batch_")
+             if not s.startswith("amr_batch_run_")]
+    return header_symbols() + batch
+
+
+def host_symbols():
+    """Entry points of the driver library: amr_run.h and the driver half of
+    amr_batch.h."""
+    return header_symbols(os.path.join(INC, "amr_run.h"), "amr_run_") + \
+        header_symbols(os.path.join(INC, "amr_batch.h"), "amr_batch_run_")
+
+
+def test_headers_declare_what_is_expected():
+    assert len(device_symbols()) == 17 + 13
+    assert "amr_batch_timestep" in device_symbols()
+    assert "amr_run_step" in host_symbols() and "amr_batch_run_step" in host_symbols()
 
 
 def test_header_matches_python_binding():
@@ -27,9 +53,17 @@ def test_library_exports_every_symbol(which):
     path = M.LIB_PATH if which == "product" else A.ORACLE_SO
     require(path, "run __graft_entry__.build()")
     lib = ctypes.CDLL(path)
-    for name in header_symbols():
+    for name in device_symbols():
         assert hasattr(lib, name), "%s does not export %s" % (path, name)
     M.load_library(path)   # argtypes for every symbol bind
+
+
+def test_driver_library_exports_every_symbol():
+    require(HOST_SO, "run __graft_entry__.build()")
+    lib = ctypes.CDLL(HOST_SO)
+    for name in host_symbols():
+        assert hasattr(lib, name), "libminiamr_host.so does not export %s" % name
+    M.load_host_library(HOST_SO)
 
 
 def test_config_struct_layout():
