@@ -14,12 +14,20 @@ per-timestep checksum read-back (40 doubles per checksum stage).
 value = cell-variable updates per second over the timed timesteps
         (updates = sum over stages of active blocks x 16^3 x 40).
 roofline: the stencil kernel.  Algorithmic bytes per launch = one read and
-        one write of every updated cell (2 x 8 B in FP64, SURVEY B_alg);
+        one write of every updated cell (2 x 8 B in FP64);
         achieved = those bytes / the stencil's device time measured with
         HIP events on the library's stream over the timed region.
 cpu_baseline: the reference's OpenMP build (oracle/_ref/miniAMR_omp.x) on
         this host, one MPI rank, on a bounded sample of the same workload
         (its first timestep, --num_tsteps 1), rate from its Summary line.
+
+--workload batch64 instead times the batching form: the 64 independent
+runs of miniamr_amd/configs/batch64.txt (8^3/12^3/16^3 blocks, 8/20/40
+variables, 2-4 refinement levels, FP32 storage) stepped together by
+amr_batch_run (persistent kernels, one graph per timestep, autotuned
+shapes), all 10 timesteps timed, next to the same 64 runs done one after
+another with the single-run driver.  The default run reports a summary of
+it under "batch_form".
 """
 import argparse
 import ctypes
@@ -139,6 +147,114 @@ def run_workload(lib, precision, schedule, warmup, steps, sums_path=None):
     return {"s0": s0, "s1": s1, "elapsed": t1 - t0, "setup_s": setup_s}
 
 
+BATCH_TABLE = os.path.join(M.PKG, "configs", "batch64.txt")
+
+
+def batch_runs():
+    with open(BATCH_TABLE) as f:
+        return [l.strip() for l in f if l.strip() and not l.startswith("#")]
+
+
+def quiet(args):
+    return args + " --report_perf 0"
+
+
+def run_batch_workload(runs, autotune=True):
+    """All timesteps of the runs as one batch; returns totals and timings."""
+    t = time.time()
+    B = M.BatchRun([quiet(r) for r in runs], autotune=autotune)
+    setup = time.time() - t
+    t0 = time.perf_counter()
+    steps = 1
+    while B.step() == 0:
+        steps += 1
+    t1 = time.perf_counter()
+    st = [B.stats(i) for i in range(B.n)]
+    out = {"updates": sum(x.cell_var_updates for x in st), "elapsed": t1 - t0,
+           "setup_s": setup, "steps": steps, "shape": B.shape(M.load_library()),
+           "tune_ms": B.tune_ms(), "device_s": B.device_ms() * 1e-3,
+           "block_stages": [x.block_stages for x in st],
+           "stages": [x.stages_done for x in st], "ebytes": PREC_BYTES[st[0].precision],
+           "precision": st[0].precision}
+    B.finish()
+    B.close()
+    return out
+
+
+def run_sequential(runs):
+    """The same runs one after another with the single-run driver."""
+    host = M.load_host_library()
+    updates = elapsed = 0.0
+    for r in runs:
+        R = M.Run(quiet(r), host=host)
+        t0 = time.perf_counter()
+        while R.step() == 0:
+            pass
+        elapsed += time.perf_counter() - t0
+        updates += R.stats().cell_var_updates
+        R.finish()
+        R.close()
+    return {"updates": updates, "elapsed": elapsed}
+
+
+def batch_summary(b, seq):
+    v = b["updates"] / b["elapsed"] / 1e9
+    sv = seq["updates"] / seq["elapsed"] / 1e9 if seq else None
+    avg = [bs / max(1, n) for bs, n in zip(b["block_stages"], b["stages"])]
+    return {
+        "value": round(v, 4), "unit": "GCell-var updates/s", "dtype": PREC_NAME[b["precision"]],
+        "runs": len(avg), "timesteps": b["steps"], "ms_per_step": round(b["elapsed"] / b["steps"] * 1e3, 3),
+        "cell_var_updates": b["updates"],
+        "active_blocks_per_run": {"min": round(min(avg), 1), "max": round(max(avg), 1),
+                                  "total": round(sum(avg), 1)},
+        "graph_device_frac_of_hbm": round(b["updates"] * 2 * b["ebytes"] / b["device_s"] / 1e9
+                                          / HBM_PEAK_GBS, 4) if b["device_s"] > 0 else None,
+        "shape": {"vpw": b["shape"][0], "stencil_wgs": b["shape"][1],
+                  "exchange_wgs": b["shape"][2], "wg_classes": b["shape"][3],
+                  "autotune_stage_ms": round(b["tune_ms"], 4)},
+        "setup_s": round(b["setup_s"], 2),
+        "sequential_value": round(sv, 4) if sv else None,
+        "speedup_vs_sequential": round(v / sv, 3) if sv else None}
+
+
+def main_batch(a):
+    runs = batch_runs()
+    b = run_batch_workload(runs)
+    seq = None if a.no_sequential else run_sequential(runs)
+    summ = batch_summary(b, seq)
+    value = summ["value"]
+    line = {
+        "metric": "GCell-var updates/s",
+        "value": value,
+        "unit": "GCell-var updates/s",
+        "n_gpus": 1,
+        "steps": b["steps"],
+        "warmup": 0,
+        "ms_per_step": summ["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": summ["dtype"],
+        "data": "synthetic: uniform [0,1) cell values, seeds 1..64",
+        "config": {"workload": "batching form: 64 independent miniAMR 27-pt runs "
+                               "(miniamr_amd/configs/batch64.txt), all 10 timesteps",
+                   "runs": summ["runs"], "active_blocks_per_run": summ["active_blocks_per_run"],
+                   "cell_var_updates": b["updates"], "shape": summ["shape"]},
+        "roofline": {"bound": "hbm",
+                     "achieved": round(b["updates"] * 2 * b["ebytes"] / b["device_s"] / 1e9, 1)
+                     if b["device_s"] > 0 else None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": summ["graph_device_frac_of_hbm"], "traffic": None,
+                     "kernel": "timestep graphs (kb_exchange x3 + kb_stencil + kb_reduce per "
+                               "stage), device time of the graph launches"},
+        "sequential": {"value": summ["sequential_value"],
+                       "speedup": summ["speedup_vs_sequential"]},
+        "setup_s": summ["setup_s"],
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def read_sums(path):
     with open(path) as f:
         return [float(x) for x in f.read().split()]
@@ -155,12 +271,18 @@ def main():
                     choices=["stage", "var"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-precision-report", action="store_true")
+    ap.add_argument("--workload", default="cfg1", choices=["cfg1", "batch64"])
+    ap.add_argument("--no-batch-report", action="store_true")
+    ap.add_argument("--no-sequential", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build runs one rank on one GPU; --gpus %d is not supported "
               "(the multi-GPU decomposition is a design, see DESIGN.md)" % a.gpus,
               file=sys.stderr)
         sys.exit(2)
+    if a.workload == "batch64":
+        main_batch(a)
+        return
     if a.warmup + a.steps > 10 or a.steps < 1:
         print("bench.py: the workload has 10 timesteps; need 1 <= steps and "
               "warmup + steps <= 10", file=sys.stderr)
@@ -240,6 +362,9 @@ def main():
                       "split_merge": round(s1.device_ms[3] - s0.device_ms[3], 2)},
         "setup_s": round(r["setup_s"], 2),
         "precision_form": precision_form,
+        "batch_form": None if a.no_batch_report else batch_summary(
+            run_batch_workload(batch_runs()),
+            None if a.no_sequential else run_sequential(batch_runs())),
         "cpu_baseline": None if a.no_cpu_baseline else cpu_baseline(),
     }
     print(json.dumps(line), flush=True)

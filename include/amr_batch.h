@@ -48,11 +48,16 @@ int amr_batch_add(amr_batch *b, const amr_dev_config *cfg, amr_dev **member);
  * e.g. once its run has finished. */
 int amr_batch_set_active(amr_batch *b, amr_dev *member, int on);
 
-/* Launch shapes: vpw variables per stencil work item (1 or 2), and the
- * number of persistent workgroups of the stencil and exchange kernels
- * (0: one wavefront per work item).  amr_batch_autotune sets them. */
-int amr_batch_set_shape(amr_batch *b, int vpw, int stencil_wgs, int exchange_wgs);
-int amr_batch_get_shape(const amr_batch *b, int shape[3]);
+/* Launch shapes: vpw variables per stencil work item (1 or 2); the number
+ * of persistent workgroups of the stencil and exchange kernels (0: as many
+ * as work items); wg_classes, bit c set: stencil items of members in block
+ * size class c are computed by a whole 256-thread workgroup instead of one
+ * wavefront (class 0: faces up to 64 cells, e.g. 8^3 blocks; 1: up to 192,
+ * e.g. 12^3; 2: up to 256, e.g. 16^3).  shape[] = {vpw, stencil_wgs,
+ * exchange_wgs, wg_classes}.  amr_batch_autotune sets them. */
+int amr_batch_set_shape(amr_batch *b, int vpw, int stencil_wgs, int exchange_wgs,
+                        int wg_classes);
+int amr_batch_get_shape(const amr_batch *b, int shape[4]);
 
 /* One stage of every member, all variables: exchange x, y, z (permuted
  * like amr_dev_comm when permute != 0), stencil, and when sum_row >= 0 the

@@ -87,7 +87,7 @@ def load_library(path=LIB_PATH):
         "amr_batch_last_error": (ctypes.c_char_p, [vp]),
         "amr_batch_add": (ip, [vp, ctypes.POINTER(Config), ctypes.POINTER(vp)]),
         "amr_batch_set_active": (ip, [vp, vp, ip]),
-        "amr_batch_set_shape": (ip, [vp, ip, ip, ip]),
+        "amr_batch_set_shape": (ip, [vp, ip, ip, ip, ip]),
         "amr_batch_get_shape": (ip, [vp, ctypes.POINTER(ctypes.c_int)]),
         "amr_batch_stage": (ip, [vp, ip, ip, ip]),
         "amr_batch_timestep": (ip, [vp, ip, ip, ip, ip]),
@@ -212,7 +212,7 @@ class BatchRun:
         return self.lib.amr_batch_run_device_ms(self.h)
 
     def shape(self, devlib):
-        sh = (ctypes.c_int * 3)()
+        sh = (ctypes.c_int * 4)()
         devlib.amr_batch_get_shape(self.lib.amr_batch_run_batch(self.h), sh)
         return list(sh)
 

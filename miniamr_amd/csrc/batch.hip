@@ -58,19 +58,22 @@ struct BMember {
    Lay L;
    int n_active, num_vars, part_stride, cls;
    int count[3], joff[3];
+   int n_items;   // stencil items: n_active * ceil(num_vars / vpw)
+   int wg;        // 1: stencil items computed by whole workgroups
 };
 
 // Shape classes of the wave-per-item stencil: columns (ny*nz) per lane and
-// plane elements ((ny+2)*(nz+2)) per lane over 64 lanes.
-//   0: ny*nz <= 64,  plane <= 128   (e.g. 8^3 blocks)
-//   1: ny*nz <= 192, plane <= 256   (e.g. 10^3, 12^3)
-//   2: ny*nz <= 256, plane <= 384   (e.g. 14^3, 16^3)
+// plane elements ((ny+2)*(nz+2)) per lane over 64 lanes, each with a
+// compile-time nz for the block size the class is meant for:
+//   0/1: ny*nz <= 64,  plane <= 128   (nz == 8 / other; e.g. 8^3)
+//   2/3: ny*nz <= 192, plane <= 256   (nz == 12 / other; e.g. 10^3, 12^3)
+//   4/5: ny*nz <= 256, plane <= 384   (nz == 16 / other; e.g. 14^3, 16^3)
 int shape_class(const Lay &L)
 {
    const int ncol = L.ny * L.nz, plane = L.gy * L.gz;
-   if (ncol <= 64 && plane <= 128) return 0;
-   if (ncol <= 192 && plane <= 256) return 1;
-   if (ncol <= 256 && plane <= 384) return 2;
+   if (ncol <= 64 && plane <= 128) return L.nz == 8 ? 0 : 1;
+   if (ncol <= 192 && plane <= 256) return L.nz == 12 ? 2 : 3;
+   if (ncol <= 256 && plane <= 384) return L.nz == 16 ? 4 : 5;
    return -1;
 }
 
@@ -107,6 +110,11 @@ __global__ __launch_bounds__(256) void kb_exchange(const BMember *__restrict__ m
    }
 }
 
+// Stencil list entries are workgroup items: for a member in workgroup mode
+// one (block, variable group) item computed by all 256 threads (the
+// single-instance kernel's shape, for large blocks); in wavefront mode four
+// consecutive items, one per wave (for small blocks, whose planes would
+// leave most of a workgroup idle).
 template <typename T, int ST, int VPW>
 __global__ __launch_bounds__(256) void kb_stencil(const BMember *__restrict__ mem,
                                                   const int *__restrict__ pre, int M,
@@ -114,27 +122,46 @@ __global__ __launch_bounds__(256) void kb_stencil(const BMember *__restrict__ me
 {
    typedef typename Acc<T>::type A;
    extern __shared__ __align__(16) unsigned char smem[];
-   A *ring = (A *) smem + (threadIdx.x >> 6) * ring_elems;
-   const int total = pre[M], stride = gridDim.x * kWavesPerWg;
-   for (int t = first_item(); t < total; t += stride) {
+   __shared__ double red[kThreads / 64];
+   const int wave = threadIdx.x >> 6;
+   A *wave_ring = (A *) smem + wave * ring_elems;
+   const int total = pre[M];
+   for (int t = blockIdx.x; t < total; t += gridDim.x) {
       const int m = find_member(pre, M, t);
       const BMember &b = mem[m];
-      const int na = b.n_active, local = t - pre[m];
-      const int g = local / na, in = local - g * na;
+      __syncthreads();   // every wave is done with the previous item's LDS
+      const int wg = b.wg;
+      const int item = wg ? t - pre[m] : (t - pre[m]) * kWavesPerWg + wave;
+      if (item >= b.n_items)
+         continue;
+      const int na = b.n_active;
+      const int g = item / na, in = item - g * na;
       const int v0 = g * VPW, nv = min(VPW, b.num_vars - v0);
       const Lay L = b.L;
       T *rec = (T *) b.pool + (int64_t) v0 * L.var_stride + (int64_t) b.sorted[in] * L.rec;
       double *part = want_partials ? b.partials + (int64_t) v0 * b.part_stride + in : nullptr;
-      switch (b.cls) {
-      case 0:
-         stencil_block<T, ST, 1, 2, VPW, 64>(rec, L, nv, ring, nullptr, part, b.part_stride);
-         break;
-      case 1:
-         stencil_block<T, ST, 3, 4, VPW, 64>(rec, L, nv, ring, nullptr, part, b.part_stride);
-         break;
-      default:
-         stencil_block<T, ST, 4, 6, VPW, 64>(rec, L, nv, ring, nullptr, part, b.part_stride);
-         break;
+      if (wg) {
+#define WITEM(NZ) \
+   stencil_block<T, ST, 1, 2, VPW, kThreads, NZ>(rec, L, nv, (A *) smem, red, part, b.part_stride)
+         switch (b.cls) {
+         case 0: WITEM(8); break;
+         case 2: WITEM(12); break;
+         case 4: WITEM(16); break;
+         default: WITEM(0); break;
+         }
+#undef WITEM
+      } else {
+#define ITEM(NC, NP, NZ) \
+   stencil_block<T, ST, NC, NP, VPW, 64, NZ>(rec, L, nv, wave_ring, nullptr, part, b.part_stride)
+         switch (b.cls) {
+         case 0: ITEM(1, 2, 8); break;
+         case 1: ITEM(1, 2, 0); break;
+         case 2: ITEM(3, 4, 12); break;
+         case 3: ITEM(3, 4, 0); break;
+         case 4: ITEM(4, 6, 16); break;
+         default: ITEM(4, 6, 0); break;
+         }
+#undef ITEM
       }
    }
 }
@@ -177,12 +204,13 @@ struct amr_batch {
    int M = 0, plane_max = 0;
    int tot_ex[3] = {0, 0, 0}, tot_st = 0, tot_rd = 0;
    bool table_ok = false;
-   int table_vpw = 0;
+   int table_vpw = 0, table_wg = -1;
    long epoch = 0;
    std::vector<int> member_rows;     // sum_rows of each table member
 
    // launch shape
    int vpw = 1, st_wgs = 0, ex_wgs = 0;
+   int wg_mask = 0;                  // bit c: size class c in workgroup mode
 
    // graph
    hipGraphExec_t gexec = nullptr;
@@ -255,7 +283,7 @@ int ceil_div(int a, int b) { return (a + b - 1) / b; }
 // a member joined or left, or the stencil's variables per item changed.
 int ensure_table(amr_batch *b)
 {
-   bool need = !b->table_ok || b->table_vpw != b->vpw;
+   bool need = !b->table_ok || b->table_vpw != b->vpw || b->table_wg != b->wg_mask;
    for (size_t i = 0; i < b->members.size() && !need; i++)
       if (b->seen[i] != b->members[i]->plan_version)
          need = true;
@@ -267,7 +295,7 @@ int ensure_table(amr_batch *b)
       if (b->active[i])
          order.push_back((int) i);
    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-      return shape_class(b->members[x]->g) > shape_class(b->members[y]->g);
+      return shape_class(b->members[x]->g) / 2 > shape_class(b->members[y]->g) / 2;
    });
    const int M = (int) order.size();
    std::vector<BMember> hm(M);
@@ -296,7 +324,9 @@ int ensure_table(amr_batch *b)
          off += d->count[dir];
          ex[dir][q + 1] = ex[dir][q] + d->count[dir] * ceil_div(m.num_vars, kVars);
       }
-      st[q + 1] = st[q] + d->n_active * ceil_div(m.num_vars, b->vpw);
+      m.n_items = d->n_active * ceil_div(m.num_vars, b->vpw);
+      m.wg = (b->wg_mask >> (m.cls / 2)) & 1;
+      st[q + 1] = st[q] + (m.wg ? m.n_items : ceil_div(m.n_items, kWavesPerWg));
       rd[q + 1] = rd[q] + m.num_vars;
       b->plane_max = std::max(b->plane_max, d->g.gy * d->g.gz);
       b->member_rows[q] = d->d_sums ? d->sum_rows : 0;
@@ -327,6 +357,7 @@ int ensure_table(amr_batch *b)
       b->seen[i] = b->members[i]->plan_version;
    b->table_ok = true;
    b->table_vpw = b->vpw;
+   b->table_wg = b->wg_mask;
    b->epoch++;
    return AMR_OK;
 }
@@ -343,7 +374,8 @@ int launch_stencil_t(amr_batch *b, int want_partials)
    typedef typename Acc<T>::type A;
    const int ring = 3 * VPW * b->plane_max;
    const size_t lds = sizeof(A) * (size_t) ring * kWavesPerWg;
-   hipLaunchKernelGGL((kb_stencil<T, ST, VPW>), dim3(grid_of(b->st_wgs, b->tot_st)), dim3(256),
+   const int grid = b->st_wgs > 0 ? std::min(b->st_wgs, b->tot_st) : b->tot_st;
+   hipLaunchKernelGGL((kb_stencil<T, ST, VPW>), dim3(grid), dim3(256),
                       lds, b->stream, b->d_mem, b->d_pre + 3 * (b->M + 1), b->M, want_partials,
                       ring);
    BHIP(b, hipGetLastError());
@@ -551,23 +583,28 @@ int amr_batch_set_active(amr_batch *b, amr_dev *member, int on)
    return berr(b, AMR_ERR_ARG, "set_active: not a member of this batch");
 }
 
-int amr_batch_set_shape(amr_batch *b, int vpw, int stencil_wgs, int exchange_wgs)
+int amr_batch_set_shape(amr_batch *b, int vpw, int stencil_wgs, int exchange_wgs,
+                        int wg_classes)
 {
-   if (!b || (vpw != 1 && vpw != 2) || stencil_wgs < 0 || exchange_wgs < 0)
-      return berr(b, AMR_ERR_ARG, "set_shape: vpw must be 1 or 2, grid sizes >= 0");
+   if (!b || (vpw != 1 && vpw != 2) || stencil_wgs < 0 || exchange_wgs < 0 ||
+       wg_classes < 0 || wg_classes > 7)
+      return berr(b, AMR_ERR_ARG, "set_shape: vpw must be 1 or 2, grid sizes >= 0, "
+                                  "wg_classes in 0..7");
    b->vpw = vpw;
    b->st_wgs = stencil_wgs;
    b->ex_wgs = exchange_wgs;
+   b->wg_mask = wg_classes;
    return AMR_OK;
 }
 
-int amr_batch_get_shape(const amr_batch *b, int shape[3])
+int amr_batch_get_shape(const amr_batch *b, int shape[4])
 {
    if (!b || !shape)
       return AMR_ERR_ARG;
    shape[0] = b->vpw;
    shape[1] = b->st_wgs;
    shape[2] = b->ex_wgs;
+   shape[3] = b->wg_mask;
    return AMR_OK;
 }
 
@@ -595,7 +632,7 @@ int amr_batch_timestep(amr_batch *b, int stage0, int nstages, int checksum_freq,
    if (checksum_freq && (rc = check_row(b, (nstages - 1) / checksum_freq)))
       return rc;
    const long key[8] = {b->epoch, permute ? stage0 % 6 : -1, nstages, checksum_freq,
-                        b->vpw, b->st_wgs, b->ex_wgs, (long) b->stencil};
+                        b->vpw, b->st_wgs, b->ex_wgs, (long) b->wg_mask};
    if (!b->gexec || memcmp(key, b->gkey, sizeof(key))) {
       drop_graph(b);
       const bool tm = b->timing;
@@ -655,39 +692,41 @@ int amr_batch_autotune(amr_batch *b, double *ms_out)
                              b->stream));
       off += p.second;
    }
+   // coordinate descent: workgroup/wavefront items per size class, then
+   // variables per stencil item, then the persistent grids
    const int reps = 3;
-   const int vpws[2] = {1, 2};
    const int mult[5] = {0, 1, 2, 4, 8};
    double best = 1e30;
-   int bv = b->vpw, bs = b->st_wgs, be = b->ex_wgs;
-   // stencil shape with the exchange fixed, then the exchange grid
-   for (int vi = 0; vi < 2 && !rc; vi++)
-      for (int si = 0; si < 5 && !rc; si++) {
-         b->vpw = vpws[vi];
-         b->st_wgs = mult[si] * b->cus;
-         b->ex_wgs = be;
-         double t = 0.0;
-         rc = time_stages(b, reps, &t);
-         if (!rc && t < best) {
-            best = t;
-            bv = b->vpw;
-            bs = b->st_wgs;
-         }
-      }
-   for (int ei = 0; ei < 5 && !rc; ei++) {
-      b->vpw = bv;
-      b->st_wgs = bs;
-      b->ex_wgs = mult[ei] * b->cus;
+   auto trial = [&](int vpw, int st, int ex, int wgm, bool *better) {
+      b->vpw = vpw;
+      b->st_wgs = st;
+      b->ex_wgs = ex;
+      b->wg_mask = wgm;
       double t = 0.0;
-      rc = time_stages(b, reps, &t);
-      if (!rc && t < best) {
+      int r = time_stages(b, reps, &t);
+      *better = !r && t < best;
+      if (*better)
          best = t;
-         be = b->ex_wgs;
-      }
-   }
+      return r;
+   };
+   int bv = b->vpw, bs = 0, be = 0, bm = 0;
+   bool better = false;
+   for (int wgm = 0; wgm < 8 && !rc; wgm++)
+      if (!(rc = trial(bv, bs, be, wgm, &better)) && better)
+         bm = wgm;
+   for (int v = 1; v <= 2 && !rc; v++)
+      if (!(rc = trial(v, bs, be, bm, &better)) && better)
+         bv = v;
+   for (int si = 1; si < 5 && !rc; si++)
+      if (!(rc = trial(bv, mult[si] * b->cus, be, bm, &better)) && better)
+         bs = mult[si] * b->cus;
+   for (int ei = 1; ei < 5 && !rc; ei++)
+      if (!(rc = trial(bv, bs, mult[ei] * b->cus, bm, &better)) && better)
+         be = mult[ei] * b->cus;
    b->vpw = bv;
    b->st_wgs = bs;
    b->ex_wgs = be;
+   b->wg_mask = bm;
    off = 0;
    for (auto &p : pools) {
       hipMemcpyAsync(p.first->pool, backup + off, p.second, hipMemcpyDeviceToDevice, b->stream);

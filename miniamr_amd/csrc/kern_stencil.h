@@ -89,15 +89,18 @@ __device__ __forceinline__ A divide(A w)
 // owned by the item; `red` 4 doubles of LDS (workgroup items only);
 // `partials`, when not null, receives the FP64 sum of the stored values of
 // variable u at partials[u * part_stride].  Requires ny*nz <= NCOL*NT and
-// (ny+2)*(nz+2) <= NPER*NT.
-template <typename T, int ST, int NCOL, int NPER, int VPW, int NT = kThreads>
+// (ny+2)*(nz+2) <= NPER*NT.  NZ, when not 0, is the block's nz known at
+// compile time (the LDS neighbour offsets and the column decomposition
+// then become constants); 0 reads it from L.
+template <typename T, int ST, int NCOL, int NPER, int VPW, int NT = kThreads, int NZ = 0>
 __device__ __forceinline__ void stencil_block(T *__restrict__ rec, const Lay &L, int nv,
                                               typename Acc<T>::type *ring, double *red,
                                               double *__restrict__ partials, int part_stride)
 {
    typedef typename Acc<T>::type A;
    constexpr int D = kDepth;           // planes of global loads in flight
-   const int plane = L.gy * L.gz, ncol = L.ny * L.nz, nplanes = L.nx + 2;
+   const int nz = NZ ? NZ : L.nz, gz = nz + 2;
+   const int plane = L.gy * gz, ncol = L.ny * nz, nplanes = L.nx + 2;
    const int64_t vs = L.var_stride;
    const int tid = NT == kThreads ? (int) threadIdx.x : (int) (threadIdx.x & (NT - 1));
 
@@ -107,11 +110,28 @@ __device__ __forceinline__ void stencil_block(T *__restrict__ rec, const Lay &L,
 #pragma unroll
    for (int q = 0; q < NPER; q++) {
       const int e = min(tid + q * NT, plane - 1);
-      const int jj = e / L.gz, kk = e - jj * L.gz;
+      const int jj = e / gz, kk = e - jj * gz;
       off0[q] = home(L, 0, jj, kk);
       offN[q] = home(L, L.nx + 1, jj, kk);
       off1[q] = home(L, 1, jj, kk);
       stp[q] = L.nx > 1 ? home(L, 2, jj, kk) - off1[q] : 0;
+   }
+   // Per output column col = (j-1)*nz + (k-1): its index in an LDS plane
+   // (-1: no such column) and the offset of its z-plane copy at x = 1 (-1:
+   // k is not 1 or nz).  The interior offset at x = i is (i-1)*ncol + col
+   // and the copy's is zc1 + (i-1)*ny.
+   int cpl[NCOL], zc1[NCOL];
+#pragma unroll
+   for (int q = 0; q < NCOL; q++) {
+      const int col = tid + q * NT;
+      cpl[q] = -1;
+      zc1[q] = -1;
+      if (col < ncol) {
+         const int j = col / nz + 1, k = col - (j - 1) * nz + 1;
+         cpl[q] = j * gz + k;
+         if (k == 1 || k == nz)
+            zc1[q] = zcopy(L, 1, j, k);
+      }
    }
    T pre[D][VPW][NPER];
 #define PREFETCH(slot, p)                                                  \
@@ -152,6 +172,8 @@ __device__ __forceinline__ void stencil_block(T *__restrict__ rec, const Lay &L,
          }
          if (p + D < nplanes) { PREFETCH(d, p + D); }
          item_sync<NT>();
+         const bool out_now = p >= 2;
+         const int i = p - 1;                  // output plane when out_now
 #pragma unroll
          for (int u = 0; u < VPW; u++) {
             if (u >= nv)
@@ -161,14 +183,12 @@ __device__ __forceinline__ void stencil_block(T *__restrict__ rec, const Lay &L,
             T *ru = rec + u * vs;
 #pragma unroll
             for (int q = 0; q < NCOL; q++) {
-               const int col = tid + q * NT;
-               if (col >= ncol)
+               const int c = cpl[q];
+               if (c < 0)
                   continue;
-               const int j = col / L.nz + 1, k = col - (j - 1) * L.nz + 1, c = j * L.gz + k;
                A w = 0;
-               bool out_now = p >= 2;
                if (ST == 27) {
-                  const A s_new = sum9(slot, c, L.gz);
+                  const A s_new = sum9(slot, c, gz);
                   if (out_now) {
                      w = (s_prev[u][q] + s_cur[u][q]) + s_new;
                      w = divide<A, 27>(w);
@@ -178,11 +198,11 @@ __device__ __forceinline__ void stencil_block(T *__restrict__ rec, const Lay &L,
                } else {
                   if (out_now) {
                      const A cc = pc[c];
-                     w = c_prev[u][q] + pc[c - L.gz];
+                     w = c_prev[u][q] + pc[c - gz];
                      w = w + pc[c - 1];
                      w = w + cc;
                      w = w + pc[c + 1];
-                     w = w + pc[c + L.gz];
+                     w = w + pc[c + gz];
                      w = w + slot[c];
                      w = divide<A, 7>(w);
                      c_prev[u][q] = cc;
@@ -192,10 +212,9 @@ __device__ __forceinline__ void stencil_block(T *__restrict__ rec, const Lay &L,
                }
                if (out_now) {
                   const T out = from_acc<T>(w);
-                  const int i = p - 1;
-                  ru[((i - 1) * L.ny + (j - 1)) * L.nz + (k - 1)] = out;
-                  if (k == 1 || k == L.nz)
-                     ru[zcopy(L, i, j, k)] = out;
+                  ru[(i - 1) * ncol + tid + q * NT] = out;
+                  if (zc1[q] >= 0)
+                     ru[zc1[q] + (i - 1) * L.ny] = out;
                   if (partials)
                      bsum[u] += (double) to_acc(out);
                }

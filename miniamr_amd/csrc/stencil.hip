@@ -32,7 +32,7 @@ namespace {
 
 using namespace amrk;
 
-template <typename T, int ST, int NCOL, int NPER, int VPW>
+template <typename T, int ST, int NCOL, int NPER, int VPW, int NZ>
 __global__ __launch_bounds__(kThreads) void k_stencil(T *__restrict__ pool,
                                                        const int32_t *__restrict__ sorted,
                                                        Lay L, int var0, int var_end,
@@ -45,7 +45,7 @@ __global__ __launch_bounds__(kThreads) void k_stencil(T *__restrict__ pool,
    const int in = blockIdx.x, v0 = var0 + blockIdx.y * VPW;
    const int nv = min(VPW, var_end - v0);
    T *rec = pool + (int64_t) v0 * L.var_stride + (int64_t) sorted[in] * L.rec;
-   stencil_block<T, ST, NCOL, NPER, VPW>(rec, L, nv, (A *) smem, red,
+   stencil_block<T, ST, NCOL, NPER, VPW, kThreads, NZ>(rec, L, nv, (A *) smem, red,
                                          partials ? partials + (int64_t) v0 * part_stride + in
                                                   : nullptr,
                                          part_stride);
@@ -128,16 +128,20 @@ int launch_st(amr_dev *d, int start, int num, double *parts)
    const int ncol = L.ny * L.nz, plane = L.gy * L.gz;
    dim3 grid(d->n_active, (num + V - 1) / V);
    size_t lds = sizeof(typename Acc<T>::type) * 3 * V * (size_t) plane;
-#define GO(NC, NP)                                                                          \
-   hipLaunchKernelGGL((k_stencil<T, ST, NC, NP, V>), grid, dim3(kThreads), lds, d->stream,  \
+#define GO(NC, NP, NZ)                                                                      \
+   hipLaunchKernelGGL((k_stencil<T, ST, NC, NP, V, NZ>), grid, dim3(kThreads), lds, d->stream, \
                       (T *) d->pool, d->d_sorted, L, start, start + num, parts,             \
                       d->c.max_blocks)
-   if (ncol <= kThreads && plane <= 2 * kThreads)
-      GO(1, 2);
-   else if (ncol <= 2 * kThreads && plane <= 4 * kThreads)
-      GO(2, 4);
+   // common face widths get a compile-time nz (constant LDS offsets)
+   if (ncol <= kThreads && plane <= 2 * kThreads) {
+      if (L.nz == 16) GO(1, 2, 16);
+      else if (L.nz == 12) GO(1, 2, 12);
+      else if (L.nz == 8) GO(1, 2, 8);
+      else GO(1, 2, 0);
+   } else if (ncol <= 2 * kThreads && plane <= 4 * kThreads)
+      GO(2, 4, 0);
    else if (ncol <= kCols * kThreads && plane <= kPer * kThreads)
-      GO(kCols, kPer);
+      GO(kCols, kPer, 0);
    else
       return amr_set_error(d, AMR_ERR_ARG, "block face %dx%d too large for the stencil kernel",
                            L.ny, L.nz);

@@ -1,7 +1,11 @@
 /*
  * batch_main.c -- miniAMR_batch.x: many independent runs on one GPU.
  *
- *   miniAMR_batch.x [--no_autotune] [--quiet] <runs file>
+ *   miniAMR_batch.x [--no_autotune] [--shape vpw,sw,ew,wc] [--quiet] <runs file>
+ *
+ * --shape fixes the launch shape (amr_batch_set_shape: variables per
+ * stencil item, stencil and exchange grid sizes, workgroup-mode classes)
+ * instead of autotuning it.
  *
  * Each non-empty line of the runs file not starting with '#' holds one
  * run's command line in the reference's syntax (the same flags as
@@ -20,18 +24,26 @@
 
 int main(int argc, char **argv)
 {
-   int autotune = 1, quiet = 0;
+   int autotune = 1, quiet = 0, shape_set = 0, shp[4] = {1, 0, 0, 0};
    const char *path = NULL;
    for (int i = 1; i < argc; i++) {
       if (!strcmp(argv[i], "--no_autotune"))
          autotune = 0;
-      else if (!strcmp(argv[i], "--quiet"))
+      else if (!strcmp(argv[i], "--shape") && i + 1 < argc) {
+         if (sscanf(argv[++i], "%d,%d,%d,%d", &shp[0], &shp[1], &shp[2], &shp[3]) != 4) {
+            printf("--shape takes vpw,stencil_wgs,exchange_wgs,wg_classes\n");
+            return -1;
+         }
+         shape_set = 1;
+         autotune = 0;
+      } else if (!strcmp(argv[i], "--quiet"))
          quiet = 1;
       else
          path = argv[i];
    }
    if (!path) {
-      printf("usage: %s [--no_autotune] [--quiet] <runs file>\n", argv[0]);
+      printf("usage: %s [--no_autotune] [--shape vpw,sw,ew,wc] [--quiet] <runs file>\n",
+             argv[0]);
       return -1;
    }
    FILE *f = fopen(path, "r");
@@ -80,6 +92,10 @@ int main(int argc, char **argv)
 
    double t0 = amr_now();
    amr_batch_run *B = amr_batch_run_create(n, argcs, argvs, autotune);
+   if (shape_set && amr_batch_set_shape(amr_batch_run_batch(B), shp[0], shp[1], shp[2], shp[3])) {
+      printf("%s\n", amr_batch_last_error(amr_batch_run_batch(B)));
+      return -1;
+   }
    double t1 = amr_now();
    int rc;
    while (!(rc = amr_batch_run_step(B)))
@@ -94,12 +110,13 @@ int main(int argc, char **argv)
       printf("run %d: ", i);
       amr_run_report(amr_batch_run_get(B, i));
    }
-   int shape[3];
+   int shape[4];
    amr_batch_get_shape(amr_batch_run_batch(B), shape);
    printf("Batch: runs %d setup %lf time %lf device %lf updates %.6e GCell-var/s %.4lf "
-          "shape vpw %d stencil_wgs %d exchange_wgs %d\n",
+          "shape vpw %d stencil_wgs %d exchange_wgs %d wg_classes %d tune_ms %.4lf\n",
           n, t1 - t0, t2 - t1, amr_batch_run_device_ms(B) * 1e-3, updates,
-          t2 > t1 ? updates / (t2 - t1) * 1e-9 : 0.0, shape[0], shape[1], shape[2]);
+          t2 > t1 ? updates / (t2 - t1) * 1e-9 : 0.0, shape[0], shape[1], shape[2], shape[3],
+          amr_batch_run_tune_ms(B));
    amr_batch_run_destroy(B);
    return rc == 2;
 }

@@ -372,7 +372,7 @@ struct amr_batch {
    amr_dev **members;
    char *active;
    int n, cap;
-   int shape[3];
+   int shape[4];
    char err[256];
 };
 
@@ -436,15 +436,17 @@ int amr_batch_set_active(amr_batch *b, amr_dev *member, int on)
    return AMR_ERR_ARG;
 }
 
-int amr_batch_set_shape(amr_batch *b, int vpw, int stencil_wgs, int exchange_wgs)
+int amr_batch_set_shape(amr_batch *b, int vpw, int stencil_wgs, int exchange_wgs,
+                        int wg_classes)
 {
    b->shape[0] = vpw;
    b->shape[1] = stencil_wgs;
    b->shape[2] = exchange_wgs;
+   b->shape[3] = wg_classes;
    return AMR_OK;
 }
 
-int amr_batch_get_shape(const amr_batch *b, int shape[3])
+int amr_batch_get_shape(const amr_batch *b, int shape[4])
 {
    memcpy(shape, b->shape, sizeof(b->shape));
    return AMR_OK;

@@ -10,7 +10,8 @@ alone:
         checksums within 1e-12 relative (the device sums in a different
         association), same Summary line;
   FP32 / BF16: against the single-run GPU driver -- cells bit-identical;
-  autotuned and untuned launch shapes: identical results.
+  every launch shape (wavefront or workgroup items, persistent grids,
+  variables per item): identical cells.
 """
 import pytest
 
@@ -51,13 +52,18 @@ def exes():
     return A.PRODUCT_BATCH_X
 
 
+# launch shapes: vpw, stencil grid, exchange grid, workgroup-mode classes
+SHAPES = {"wave": "1,0,0,0", "wg_all_persistent": "2,512,256,7", "mixed": "1,1024,0,5",
+          "tuned": None}
+
+
 @pytest.mark.parametrize("common", [COMMON27, COMMON7], ids=["27pt", "7pt_permute"])
-@pytest.mark.parametrize("tune", [False, True], ids=["fixed", "autotuned"])
-def test_batch_f64_matches_oracle(exes, common, tune):
-    print("case: batch f64 %s tune=%s" % (common.split()[1], tune), flush=True)
+@pytest.mark.parametrize("shape", sorted(SHAPES))
+def test_batch_f64_matches_oracle(exes, common, shape):
+    print("case: batch f64 %s shape=%s" % (common.split()[1], shape), flush=True)
     runs = [r + common for r in RUNS]
     b = A.run_batch(exes, runs, dump="cells", timeout=300,
-                    flags=() if tune else ("--no_autotune",))
+                    flags=("--shape", SHAPES[shape]) if SHAPES[shape] else ())
     assert b["rc"] == 0, b["stdout"][-2000:]
     for i, args in enumerate(runs):
         s = A.run(A.ORACLE_X, args, dump="cells", timeout=300)
@@ -79,11 +85,15 @@ def test_batch_narrow_matches_single_runs(exes, prec):
         print("run %d: %s" % (i, rep), flush=True)
 
 
-def test_batch_shapes_do_not_change_results(exes):
-    print("case: batch f32 tuned vs fixed", flush=True)
+@pytest.mark.parametrize("shape", ["wave", "wg_all_persistent", "mixed"])
+def test_batch_shapes_do_not_change_results(exes, shape):
+    """Cells of the FP32 batch do not depend on the launch shape (the same
+    item bodies run in every shape); checksums only through the association
+    of the per-block partial sums (workgroup vs wavefront items)."""
+    print("case: batch f32 tuned vs %s" % shape, flush=True)
     runs = [r + COMMON27 + " --precision f32" for r in RUNS]
     a = A.run_batch(exes, runs, dump="cells", timeout=300)
-    c = A.run_batch(exes, runs, dump="cells", timeout=300, flags=("--no_autotune",))
+    c = A.run_batch(exes, runs, dump="cells", timeout=300, flags=("--shape", SHAPES[shape]))
     assert a["rc"] == 0 and c["rc"] == 0
     for i in range(len(runs)):
-        A.same_run(a["runs"][i], c["runs"][i], sum_rtol=0.0, max_ulp=0)
+        A.same_run(a["runs"][i], c["runs"][i], sum_rtol=1e-13, max_ulp=0)
