@@ -67,7 +67,7 @@ def load_host():
     path = os.path.join(M.PKG, "lib", "libminiamr_host.so")
     if not os.path.exists(path):
         raise M.AmrError("%s is not built: run `make -C miniamr_amd`" % path)
-    lib = ctypes.CDLL(path)
+    lib = M._cdll(path)
     lib.amr_run_create.restype = ctypes.c_void_p
     lib.amr_run_create.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p)]
     for name in ("amr_run_step", "amr_run_finish"):

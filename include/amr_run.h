@@ -51,6 +51,12 @@ int amr_run_finish(amr_run *run);
 int amr_run_stats_get(const amr_run *run, amr_run_stats *out);
 void amr_run_report(amr_run *run);
 amr_dev *amr_run_device(amr_run *run);
+/* Active blocks now, and block `in` (0 <= in < amr_run_active) of the
+ * sorted list (ref/block.h:72-77): slot, level and extent [lo, hi] in the
+ * unit domain.  Returns 0, or -1 for a bad index. */
+int amr_run_active(const amr_run *run);
+int amr_run_block(const amr_run *run, int in, int32_t *slot, int32_t *level, double lo[3],
+                  double hi[3]);
 void amr_run_destroy(amr_run *run);
 
 #ifdef __cplusplus

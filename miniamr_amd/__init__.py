@@ -55,12 +55,27 @@ class AmrError(RuntimeError):
     pass
 
 
+# One HIP runtime per process.  torch ships its own libamdhip64 with the
+# same soname as ROCm's: imported first, it is the runtime the libraries
+# bind to; imported after a library was loaded, torch would load a second
+# copy next to ROCm's.  _LOADED_BEFORE_TORCH records that second order so
+# that miniamr_amd.autodiff (which needs torch) can refuse it.
+_LOADED_BEFORE_TORCH = []
+
+
+def _cdll(path):
+    import sys
+    if "torch" not in sys.modules:
+        _LOADED_BEFORE_TORCH.append(path)
+    return ctypes.CDLL(path)
+
+
 def load_library(path=LIB_PATH):
     """Load an implementation of amr_dev.h (the HIP library by default)."""
     if not os.path.exists(path):
         raise AmrError("%s is not built: run `make -C miniamr_amd` (or "
                        "__graft_entry__.build())" % path)
-    lib = ctypes.CDLL(path)
+    lib = _cdll(path)
     vp, ip, i32p, dp = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int32), \
         ctypes.POINTER(ctypes.c_double)
     sig = {
@@ -95,9 +110,36 @@ def load_library(path=LIB_PATH):
         "amr_batch_sync": (ip, [vp]),
         "amr_batch_set_timing": (ip, [vp, ip]),
         "amr_batch_read_timing": (ip, [vp, dp]),
+        # include/amr_adjoint.h
+        "amr_dev_tape": (ip, [vp, ip]),
+        "amr_dev_tape_length": (ip, [vp, ctypes.POINTER(ctypes.c_int)]),
+        "amr_dev_tape_op": (ip, [vp, ip, i32p]),
+        "amr_dev_tape_plan": (ip, [vp, ip, i32p, i32p, i32p, vp]),
+        "amr_dev_tape_split_jobs": (ip, [vp, ip, vp]),
+        "amr_dev_tape_merge_jobs": (ip, [vp, ip, vp]),
+        "amr_dev_replay": (ip, [vp, ip, ip]),
+        "amr_dev_state_bytes": (ip, [vp, ctypes.POINTER(ctypes.c_size_t)]),
+        "amr_dev_save_state": (ip, [vp, vp]),
+        "amr_dev_load_state": (ip, [vp, vp]),
+        "amr_dev_clear_block": (ip, [vp, ip]),
+        "amr_adj_create": (ip, [vp, ctypes.POINTER(vp)]),
+        "amr_adj_destroy": (ip, [vp]),
+        "amr_adj_last_error": (ctypes.c_char_p, [vp]),
+        "amr_adj_zero": (ip, [vp]),
+        "amr_adj_load": (ip, [vp, vp]),
+        "amr_adj_save": (ip, [vp, vp]),
+        "amr_adj_get_block": (ip, [vp, ip, dp]),
+        "amr_adj_set_block": (ip, [vp, ip, dp]),
+        "amr_adj_run": (ip, [vp, ip, ip]),
+        "amr_adj_sync": (ip, [vp]),
     }
     for name, (res, args) in sig.items():
-        f = getattr(lib, name)
+        f = getattr(lib, name, None)
+        if f is None:
+            if name.startswith(("amr_adj", "amr_dev_tape_", "amr_dev_replay", "amr_dev_state",
+                                "amr_dev_save", "amr_dev_load", "amr_dev_clear")):
+                continue   # reverse mode: the HIP library only
+            raise AmrError("%s does not export %s" % (path, name))
         f.restype = res
         f.argtypes = args
     return lib
@@ -122,7 +164,7 @@ def load_host_library(path=HOST_LIB_PATH):
     include/amr_batch.h)."""
     if not os.path.exists(path):
         raise AmrError("%s is not built: run `make -C miniamr_amd`" % path)
-    lib = ctypes.CDLL(path)
+    lib = _cdll(path)
     vp, ip, dp = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
     argv_t = ctypes.POINTER(ctypes.c_char_p)
     sig = {
@@ -132,6 +174,10 @@ def load_host_library(path=HOST_LIB_PATH):
         "amr_run_stats_get": (ip, [vp, ctypes.POINTER(RunStats)]),
         "amr_run_report": (None, [vp]),
         "amr_run_device": (vp, [vp]),
+        "amr_run_active": (ip, [vp]),
+        "amr_run_block": (ip, [vp, ip, ctypes.POINTER(ctypes.c_int32),
+                                ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(dp),
+                                ctypes.POINTER(dp)]),
         "amr_run_destroy": (None, [vp]),
         "amr_batch_run_create": (vp, [ip, ctypes.POINTER(ctypes.c_int),
                                       ctypes.POINTER(argv_t), ip]),
@@ -176,6 +222,19 @@ class Run:
 
     def finish(self):
         return self.lib.amr_run_finish(self.h)
+
+    def device(self):
+        return self.lib.amr_run_device(self.h)
+
+    def blocks(self):
+        """Active blocks in sorted order: list of (slot, level, lo[3], hi[3])."""
+        out = []
+        for i in range(self.lib.amr_run_active(self.h)):
+            slot, level = ctypes.c_int32(), ctypes.c_int32()
+            lo, hi = (ctypes.c_double * 3)(), (ctypes.c_double * 3)()
+            self.lib.amr_run_block(self.h, i, ctypes.byref(slot), ctypes.byref(level), lo, hi)
+            out.append((slot.value, level.value, list(lo), list(hi)))
+        return out
 
     def close(self):
         if self.h:

@@ -28,6 +28,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <array>
 #include <vector>
 #include "amr_dev.h"
 
@@ -40,6 +41,22 @@ struct Lay {
    int oZI, oZX, oZY;       // z = 1 / z = nz copies
    int64_t rec;             // elements per (var, slot), padded to 256 bytes
    int64_t var_stride;      // rec * max_blocks
+};
+
+// Recorded data operations of a context (include/amr_adjoint.h): each op
+// is {kind, args..., plan in force}; plans and job lists are host copies.
+struct TapePlan {
+   std::vector<int32_t> sorted;
+   std::vector<amr_face_job> jobs;
+   int32_t count[3];
+};
+struct Tape {
+   bool on = false;
+   std::vector<std::array<int32_t, 8>> ops;
+   std::vector<TapePlan> plans;
+   std::vector<amr_split_job> splits;
+   std::vector<amr_merge_job> merges;
+   int cur_plan = -1;
 };
 
 struct amr_dev {
@@ -68,6 +85,9 @@ struct amr_dev {
    size_t scratch_bytes = 0;
    void *d_jobbuf = nullptr;        // split/merge job array
    size_t jobbuf_bytes = 0;
+
+   Tape *tape = nullptr;            // recording of data operations, if any
+   bool replaying = false;          // calls come from the tape: do not record
 
    bool timing = false;
    std::vector<hipEvent_t> ev_free;
@@ -154,6 +174,19 @@ int amr_hip_check(amr_dev *d, hipError_t e, const char *what);
 // 3 split/merge.
 void amr_span_begin(amr_dev *d, int cat);
 void amr_span_end(amr_dev *d);
+
+// Unrecorded implementations behind the public calls (context.hip), used
+// by the tape's replay (adjoint.hip).
+int amr_set_mesh_impl(amr_dev *d, const int32_t *sorted, int n_active, const amr_face_job *jobs,
+                      const int32_t count[3]);
+int amr_comm_impl(amr_dev *d, int start, int num, int stage, int permute);
+int amr_stencil_impl(amr_dev *d, int var);
+int amr_stage_impl(amr_dev *d, int start, int num, int stage, int permute, int sum_row);
+int amr_split_impl(amr_dev *d, const amr_split_job *jobs, int n);
+int amr_merge_impl(amr_dev *d, const amr_merge_job *jobs, int n);
+// Append an operation to the tape when recording (adjoint.hip).
+void amr_tape_record(amr_dev *d, int kind, int a1 = 0, int a2 = 0, int a3 = 0, int a4 = 0);
+void amr_tape_free(amr_dev *d);
 
 // Launchers implemented per source file; all enqueue on d->stream.
 int amr_launch_exchange_dir(amr_dev *d, int dir, int start, int num);

@@ -5,6 +5,7 @@
 #include <string.h>
 #include <algorithm>
 
+#include "amr_adjoint.h"
 #include "amr_internal.h"
 
 int amr_set_error(amr_dev *d, int rc, const char *fmt, ...)
@@ -154,6 +155,7 @@ int amr_dev_destroy(amr_dev *d)
       hipHostFree(d->h_pinned);
    if (d->stream && d->owns_stream)
       hipStreamDestroy(d->stream);
+   amr_tape_free(d);
    delete d;
    return AMR_OK;
 }
@@ -173,6 +175,7 @@ int amr_dev_upload_block(amr_dev *d, int slot, const double *cells)
    if (rc)
       return rc;
    HIP_TRY(d, hipStreamSynchronize(d->stream));
+   amr_tape_record(d, AMR_OP_UPLOAD, slot);
    return AMR_OK;
 }
 
@@ -189,8 +192,10 @@ int amr_dev_download_block(amr_dev *d, int slot, double *cells)
    return AMR_OK;
 }
 
-int amr_dev_set_mesh(amr_dev *d, const int32_t *sorted, int n_active,
-                     const amr_face_job *jobs, const int32_t count[3])
+}  // extern "C"
+
+int amr_set_mesh_impl(amr_dev *d, const int32_t *sorted, int n_active,
+                      const amr_face_job *jobs, const int32_t count[3])
 {
    if (!d || n_active < 0 || n_active > d->c.max_blocks || (n_active && !sorted) || !count)
       return amr_set_error(d, AMR_ERR_ARG, "set_mesh: bad arguments");
@@ -235,7 +240,7 @@ int amr_dev_set_mesh(amr_dev *d, const int32_t *sorted, int n_active,
    return AMR_OK;
 }
 
-int amr_dev_comm(amr_dev *d, int start, int num, int stage, int permute)
+int amr_comm_impl(amr_dev *d, int start, int num, int stage, int permute)
 {
    static const int perm[6][3] = { {0, 1, 2}, {1, 2, 0}, {2, 0, 1},
                                    {0, 2, 1}, {1, 0, 2}, {2, 1, 0} };
@@ -255,13 +260,48 @@ int amr_dev_comm(amr_dev *d, int start, int num, int stage, int permute)
    return AMR_OK;
 }
 
-int amr_dev_stencil(amr_dev *d, int var)
+int amr_stencil_impl(amr_dev *d, int var)
 {
    if (!d || var < 0 || var >= d->c.num_vars)
       return amr_set_error(d, AMR_ERR_ARG, "stencil: variable %d out of range", var);
    amr_span_begin(d, 1);
    int rc = amr_launch_stencil(d, var, 1, false);
    amr_span_end(d);
+   return rc;
+}
+
+extern "C" {
+
+int amr_dev_set_mesh(amr_dev *d, const int32_t *sorted, int n_active,
+                     const amr_face_job *jobs, const int32_t count[3])
+{
+   int rc = amr_set_mesh_impl(d, sorted, n_active, jobs, count);
+   if (rc || !d->tape || !d->tape->on || d->replaying)
+      return rc;
+   TapePlan pl;
+   pl.sorted.assign(sorted, sorted + n_active);
+   pl.jobs.assign(jobs, jobs + (count[0] + count[1] + count[2]));
+   for (int i = 0; i < 3; i++)
+      pl.count[i] = count[i];
+   d->tape->plans.push_back(std::move(pl));
+   d->tape->cur_plan = (int) d->tape->plans.size() - 1;
+   amr_tape_record(d, AMR_OP_SET_MESH, d->tape->cur_plan);
+   return AMR_OK;
+}
+
+int amr_dev_comm(amr_dev *d, int start, int num, int stage, int permute)
+{
+   int rc = amr_comm_impl(d, start, num, stage, permute);
+   if (!rc)
+      amr_tape_record(d, AMR_OP_COMM, start, num, stage, permute);
+   return rc;
+}
+
+int amr_dev_stencil(amr_dev *d, int var)
+{
+   int rc = amr_stencil_impl(d, var);
+   if (!rc)
+      amr_tape_record(d, AMR_OP_STENCIL, var);
    return rc;
 }
 
@@ -355,16 +395,38 @@ static int refine_data(amr_dev *d, const J *jobs, int n, bool split)
    return rc;
 }
 
-extern "C" {
-
-int amr_dev_split(amr_dev *d, const amr_split_job *jobs, int n)
+int amr_split_impl(amr_dev *d, const amr_split_job *jobs, int n)
 {
    return refine_data(d, jobs, n, true);
 }
 
-int amr_dev_consolidate(amr_dev *d, const amr_merge_job *jobs, int n)
+int amr_merge_impl(amr_dev *d, const amr_merge_job *jobs, int n)
 {
    return refine_data(d, jobs, n, false);
+}
+
+extern "C" {
+
+int amr_dev_split(amr_dev *d, const amr_split_job *jobs, int n)
+{
+   int rc = refine_data(d, jobs, n, true);
+   if (!rc && n && d->tape && d->tape->on && !d->replaying) {
+      const int first = (int) d->tape->splits.size();
+      d->tape->splits.insert(d->tape->splits.end(), jobs, jobs + n);
+      amr_tape_record(d, AMR_OP_SPLIT, first, n);
+   }
+   return rc;
+}
+
+int amr_dev_consolidate(amr_dev *d, const amr_merge_job *jobs, int n)
+{
+   int rc = refine_data(d, jobs, n, false);
+   if (!rc && n && d->tape && d->tape->on && !d->replaying) {
+      const int first = (int) d->tape->merges.size();
+      d->tape->merges.insert(d->tape->merges.end(), jobs, jobs + n);
+      amr_tape_record(d, AMR_OP_MERGE, first, n);
+   }
+   return rc;
 }
 
 int amr_dev_reserve_sums(amr_dev *d, int n_rows)
@@ -404,13 +466,15 @@ int amr_dev_read_sums(amr_dev *d, int row0, int n_rows, double *out)
    return AMR_OK;
 }
 
-int amr_dev_stage(amr_dev *d, int start, int num, int stage, int permute, int sum_row)
+}  // extern "C"
+
+int amr_stage_impl(amr_dev *d, int start, int num, int stage, int permute, int sum_row)
 {
    if (!d || start < 0 || num < 0 || start + num > d->c.num_vars)
       return amr_set_error(d, AMR_ERR_ARG, "stage: variables out of range");
    if (sum_row >= d->sum_rows)
       return amr_set_error(d, AMR_ERR_ARG, "stage: checksum row %d not reserved", sum_row);
-   int rc = amr_dev_comm(d, start, num, stage, permute);
+   int rc = amr_comm_impl(d, start, num, stage, permute);
    if (rc)
       return rc;
    amr_span_begin(d, 1);
@@ -422,6 +486,16 @@ int amr_dev_stage(amr_dev *d, int start, int num, int stage, int permute, int su
    rc = amr_launch_reduce_sums(d, start, num,
                                d->d_sums + (size_t) sum_row * d->c.num_vars + start);
    amr_span_end(d);
+   return rc;
+}
+
+extern "C" {
+
+int amr_dev_stage(amr_dev *d, int start, int num, int stage, int permute, int sum_row)
+{
+   int rc = amr_stage_impl(d, start, num, stage, permute, sum_row);
+   if (!rc)
+      amr_tape_record(d, AMR_OP_STAGE, start, num, stage, permute);
    return rc;
 }
 

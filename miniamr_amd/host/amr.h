@@ -48,6 +48,7 @@ typedef struct {
    int device;
    int seed;             /* --seed: generator seed of the initial values
                             (1 = the reference's implicit srand(1)) */
+   int tape;             /* --tape: record data operations from allocation on */
 } amr_params;
 
 /* A block slot (ref/block.h:36-53 minus off-rank fields). */

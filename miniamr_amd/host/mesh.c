@@ -25,6 +25,7 @@
 #include <time.h>
 
 #include "amr.h"
+#include "amr_adjoint.h"
 #include "amr_batch.h"
 
 void amr_fatal(const char *fmt, ...)
@@ -117,6 +118,8 @@ int amr_allocate(amr_run *R)
    if (rc != AMR_OK)
       amr_fatal("ERROR: could not create the device context (%d): %s\n", rc,
                 R->dev ? amr_dev_last_error(R->dev) : "no context");
+   if (P->tape)
+      amr_dev_tape(R->dev, 1);
    return 0;
 }
 

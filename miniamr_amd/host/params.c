@@ -77,6 +77,7 @@ void amr_print_help(void)
       "Extensions of this build (not in the reference):",
       "--seed - seed of the initial values' generator (default 1, the reference's)",
       "--precision - storage of the cell values: f64 (default), f32 or bf16",
+      "--tape - record the device's data operations for the reverse pass (amr_adjoint.h)",
       "",
       "Environment: MINIAMR_PRECISION=f64|f32|bf16 storage precision (default f64),",
       "             MINIAMR_SCHEDULE=stage|var stage-major or variable-major order,",
@@ -134,7 +135,7 @@ int amr_parse_args(amr_params *P, int argc, char **argv)
       {"--blocking_send", &P->nonblocking, 0}, {"--refine_ghost", &P->refine_ghost, 1},
       {"--change_dir", &P->change_dir, 1}, {"--group_blocks", &P->group_blocks, 1},
       {"--break_ties", &P->group_blocks, 2}, {"--send_faces", &P->send_faces, 1},
-      {"--rcb", &P->lb_method, 0}, {"--morton", &P->lb_method, 1},
+      {"--rcb", &P->lb_method, 0}, {"--morton", &P->lb_method, 1}, {"--tape", &P->tape, 1},
       {"--hilbert", &P->lb_method, 2}, {"--trunc_hilbert", &P->lb_method, 3},
       {NULL, NULL, 0} };
    int object_num = 0;

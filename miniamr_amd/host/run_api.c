@@ -62,6 +62,34 @@ amr_dev *amr_run_device(amr_run *R)
    return R->dev;
 }
 
+int amr_run_active(const amr_run *R)
+{
+   return amr_n_active(R);
+}
+
+/* Block `in` of the sorted list: its slot, level and extent in the unit
+ * domain (the corners ref/move.c:80-88 computes). */
+int amr_run_block(const amr_run *R, int in, int32_t *slot, int32_t *level, double lo[3],
+                  double hi[3])
+{
+   if (in < 0 || in >= amr_n_active(R))
+      return -1;
+   int s = R->sorted_slot[in];
+   const amr_block *b = &R->blocks[s];
+   int sz = R->p2[R->P.num_refine - b->level];
+   if (slot)
+      *slot = s;
+   if (level)
+      *level = b->level;
+   for (int d = 0; d < 3; d++) {
+      if (lo)
+         lo[d] = ((double) (b->cen[d] - sz)) / ((double) R->mesh_size[d]);
+      if (hi)
+         hi[d] = ((double) (b->cen[d] + sz)) / ((double) R->mesh_size[d]);
+   }
+   return 0;
+}
+
 void amr_run_destroy(amr_run *R)
 {
    if (!R)

@@ -491,3 +491,14 @@ int amr_batch_read_timing(amr_batch *b, double ms[4])
    ms[0] = ms[1] = ms[2] = ms[3] = 0.0;
    return AMR_OK;
 }
+
+/* include/amr_adjoint.h: the reverse pass has its own checker (the torch
+ * restatement used by the tests); the CPU oracle records nothing. */
+#include "amr_adjoint.h"
+
+int amr_dev_tape(amr_dev *d, int on)
+{
+   if (on)
+      return fail(d, AMR_ERR_STATE, "the CPU oracle does not record a tape");
+   return AMR_OK;
+}

@@ -3,6 +3,11 @@ import sys
 
 import pytest
 
+# torch (used by the reverse-mode tests) brings its own HIP runtime with the
+# same soname as ROCm's; importing it before any library is loaded keeps a
+# single runtime in the test process.
+import torch  # noqa: F401,E402
+
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 for p in (REPO, os.path.join(REPO, "tests")):
     if p not in sys.path:

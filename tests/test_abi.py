@@ -31,6 +31,13 @@ def device_symbols():
     return header_symbols() + batch
 
 
+def adjoint_symbols():
+    """include/amr_adjoint.h: the HIP library only (the CPU oracle has just
+    amr_dev_tape, which refuses)."""
+    h = os.path.join(INC, "amr_adjoint.h")
+    return header_symbols(h, "amr_dev_") + header_symbols(h, "amr_adj_")
+
+
 def host_symbols():
     """Entry points of the driver library: amr_run.h and the driver half of
     amr_batch.h."""
@@ -40,6 +47,7 @@ def host_symbols():
 
 def test_headers_declare_what_is_expected():
     assert len(device_symbols()) == 17 + 13
+    assert len(adjoint_symbols()) == 11 + 10
     assert "amr_batch_timestep" in device_symbols()
     assert "amr_run_step" in host_symbols() and "amr_batch_run_step" in host_symbols()
 
@@ -54,6 +62,8 @@ def test_library_exports_every_symbol(which):
     require(path, "run __graft_entry__.build()")
     lib = ctypes.CDLL(path)
     for name in device_symbols():
+        assert hasattr(lib, name), "%s does not export %s" % (path, name)
+    for name in adjoint_symbols() if which == "product" else ["amr_dev_tape"]:
         assert hasattr(lib, name), "%s does not export %s" % (path, name)
     M.load_library(path)   # argtypes for every symbol bind
 
