@@ -209,9 +209,7 @@ def batch_summary(b, seq):
                                   "total": round(sum(avg), 1)},
         "graph_device_frac_of_hbm": round(b["updates"] * 2 * b["ebytes"] / b["device_s"] / 1e9
                                           / HBM_PEAK_GBS, 4) if b["device_s"] > 0 else None,
-        "shape": {"vpw": b["shape"][0], "stencil_wgs": b["shape"][1],
-                  "exchange_wgs": b["shape"][2], "wg_classes": b["shape"][3],
-                  "autotune_stage_ms": round(b["tune_ms"], 4)},
+        "shape": dict(b["shape"], autotune_stage_ms=round(b["tune_ms"], 4)),
         "setup_s": round(b["setup_s"], 2),
         "sequential_value": round(sv, 4) if sv else None,
         "speedup_vs_sequential": round(v / sv, 3) if sv else None}

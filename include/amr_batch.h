@@ -8,9 +8,10 @@
  * pool and mesh, and every amr_dev_* call works on it as usual (upload,
  * set_mesh, split, consolidate, checksum table, download).  What the batch
  * adds is stepping all members at once: one persistent kernel per stage
- * phase (x, y, z exchange; stencil; checksum reduce) walks a flat list of
- * work items drawn from every member, so small meshes share the GPU instead
- * of each launching its own underfilled grids.  A whole timestep of such
+ * phase (x, y, z exchange; stencil, one kernel per block-size class
+ * present; checksum reduce) walks a flat list of work items drawn from
+ * every member, so small meshes share the GPU instead of each launching
+ * its own underfilled grids.  A whole timestep of such
  * launches is captured once in a hipGraph and replayed until a member's
  * mesh changes; the launch shapes (persistent grid size, variables per work
  * item) come from amr_batch_autotune, which times candidates on the actual
@@ -48,16 +49,22 @@ int amr_batch_add(amr_batch *b, const amr_dev_config *cfg, amr_dev **member);
  * e.g. once its run has finished. */
 int amr_batch_set_active(amr_batch *b, amr_dev *member, int on);
 
-/* Launch shapes: vpw variables per stencil work item (1 or 2); the number
- * of persistent workgroups of the stencil and exchange kernels (0: as many
- * as work items); wg_classes, bit c set: stencil items of members in block
- * size class c are computed by a whole 256-thread workgroup instead of one
- * wavefront (class 0: faces up to 64 cells, e.g. 8^3 blocks; 1: up to 192,
- * e.g. 12^3; 2: up to 256, e.g. 16^3).  shape[] = {vpw, stencil_wgs,
- * exchange_wgs, wg_classes}.  amr_batch_autotune sets them. */
-int amr_batch_set_shape(amr_batch *b, int vpw, int stencil_wgs, int exchange_wgs,
-                        int wg_classes);
-int amr_batch_get_shape(const amr_batch *b, int shape[4]);
+/* Launch shape of the batch's kernels (amr_batch_autotune chooses one). */
+typedef struct {
+   int vpw;            /* variables per stencil work item: 1 or 2 */
+   int stencil_wgs;    /* persistent workgroups of a stencil kernel; 0: one
+                          per work item */
+   int exchange_wgs;   /* the same for the exchange kernels */
+   int wg_classes;     /* bit c set: stencil items of block size class c are
+                          computed by a whole 256-thread workgroup instead of
+                          one wavefront (class 0: faces up to 64 cells, e.g.
+                          8^3 blocks; 1: up to 192, e.g. 12^3; 2: up to 256,
+                          e.g. 16^3) */
+   int exchange_vars;  /* variables per exchange work item: 4, 8 or 16 */
+} amr_batch_shape;
+
+int amr_batch_set_shape(amr_batch *b, const amr_batch_shape *shape);
+int amr_batch_get_shape(const amr_batch *b, amr_batch_shape *shape);
 
 /* One stage of every member, all variables: exchange x, y, z (permuted
  * like amr_dev_comm when permute != 0), stencil, and when sum_row >= 0 the

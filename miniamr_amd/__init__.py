@@ -51,6 +51,13 @@ SPLIT_JOB = np.dtype([("parent", "<i4"), ("child", "<i4"), ("octant", "<i4"), ("
 MERGE_JOB = np.dtype([("dst", "<i4"), ("child", "<i4"), ("octant", "<i4"), ("pad", "<i4")])
 
 
+class BatchShape(ctypes.Structure):
+    """amr_batch_shape of include/amr_batch.h."""
+    _fields_ = [("vpw", ctypes.c_int), ("stencil_wgs", ctypes.c_int),
+                ("exchange_wgs", ctypes.c_int), ("wg_classes", ctypes.c_int),
+                ("exchange_vars", ctypes.c_int)]
+
+
 class AmrError(RuntimeError):
     pass
 
@@ -102,8 +109,8 @@ def load_library(path=LIB_PATH):
         "amr_batch_last_error": (ctypes.c_char_p, [vp]),
         "amr_batch_add": (ip, [vp, ctypes.POINTER(Config), ctypes.POINTER(vp)]),
         "amr_batch_set_active": (ip, [vp, vp, ip]),
-        "amr_batch_set_shape": (ip, [vp, ip, ip, ip, ip]),
-        "amr_batch_get_shape": (ip, [vp, ctypes.POINTER(ctypes.c_int)]),
+        "amr_batch_set_shape": (ip, [vp, ctypes.POINTER(BatchShape)]),
+        "amr_batch_get_shape": (ip, [vp, ctypes.POINTER(BatchShape)]),
         "amr_batch_stage": (ip, [vp, ip, ip, ip]),
         "amr_batch_timestep": (ip, [vp, ip, ip, ip, ip]),
         "amr_batch_autotune": (ip, [vp, dp]),
@@ -271,9 +278,9 @@ class BatchRun:
         return self.lib.amr_batch_run_device_ms(self.h)
 
     def shape(self, devlib):
-        sh = (ctypes.c_int * 4)()
-        devlib.amr_batch_get_shape(self.lib.amr_batch_run_batch(self.h), sh)
-        return list(sh)
+        sh = BatchShape()
+        devlib.amr_batch_get_shape(self.lib.amr_batch_run_batch(self.h), ctypes.byref(sh))
+        return {f: getattr(sh, f) for f, _ in BatchShape._fields_}
 
     def finish(self):
         return self.lib.amr_batch_run_finish(self.h)

@@ -15,9 +15,10 @@
 //   * sources are the elements the forward read (homes, or for the z
 //     exchange the copies), so the copies' adjoints flow back through the
 //     writes that filled them.
-// Within one exchange phase no element is read by two fills nor both read
-// and written, and no two split (merge) jobs of one reverse segment touch
-// a common slot, so the "+=" need no atomics.
+// Within one exchange phase no element is both read and written, and no
+// two split (merge) jobs of one reverse segment touch a common slot; the
+// "+=" onto sources are atomic adds (FP64 and FP32 storage: reverse mode
+// does not support BF16).
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
@@ -158,10 +159,13 @@ __device__ __forceinline__ typename Acc<T>::type take(T *rec, int h, int c)
    return mu;
 }
 
+// Accumulate into a source element.  Atomic: a real mesh never lets two
+// fills of one phase read the same element, but the adjoint does not rely
+// on it (any plan the forward accepts is transposed correctly).
 template <typename T>
 __device__ __forceinline__ void give(T *rec, int o, typename Acc<T>::type v)
 {
-   rec[o] = from_acc<T>(to_acc(rec[o]) + v);
+   atomicAdd(&rec[o], v);
 }
 
 template <typename T>
@@ -338,6 +342,13 @@ int by_precision(int prec, F f)
    }
 }
 
+// Adjoint contexts exist for FP64 and FP32 storage only.
+template <typename F>
+int adj_precision(int prec, F f)
+{
+   return prec == AMR_PREC_F64 ? f((double) 0) : f((float) 0);
+}
+
 int load_plan(amr_adj *a, int plan)
 {
    if (plan == a->plan)
@@ -381,7 +392,7 @@ int adj_stencil(amr_adj *a, int var)
    amr_dev *d = a->d;
    if (!a->n_active)
       return AMR_OK;
-   return by_precision(d->c.precision, [&](auto z) -> int {
+   return adj_precision(d->c.precision, [&](auto z) -> int {
       typedef decltype(z) T;
       typedef typename Acc<T>::type A;
       const size_t lds = sizeof(A) * (size_t) d->g.gx * d->g.gy * d->g.gz;
@@ -407,7 +418,7 @@ int adj_comm(amr_adj *a, int start, int num, int stage, int permute)
       const int n = a->count[dir];
       if (!n || !num)
          continue;
-      int rc = by_precision(d->c.precision, [&](auto z) -> int {
+      int rc = adj_precision(d->c.precision, [&](auto z) -> int {
          typedef decltype(z) T;
          hipLaunchKernelGGL(k_adj_exchange<T>, dim3(n, num), dim3(128), 0, d->stream,
                             (T *) a->pool, a->d_jobs + off, d->g, start);
@@ -451,7 +462,7 @@ int adj_refine(amr_adj *a, const J *jobs, int n)
          return rc;
       AHIP(a, hipStreamSynchronize(d->stream));
       AHIP(a, hipMemcpy(a->d_jobbuf, seg.data(), sizeof(J) * seg.size(), hipMemcpyHostToDevice));
-      rc = by_precision(d->c.precision, [&](auto z) -> int {
+      rc = adj_precision(d->c.precision, [&](auto z) -> int {
          typedef decltype(z) T;
          dim3 grid((unsigned) seg.size(), d->c.num_vars);
          if (SPLIT)
@@ -647,6 +658,8 @@ int amr_adj_create(amr_dev *d, amr_adj **out)
    if (!d)
       return aerr(a, AMR_ERR_ARG, "no forward context");
    a->d = d;
+   if (d->c.precision == AMR_PREC_BF16)
+      return aerr(a, AMR_ERR_ARG, "reverse mode supports FP64 and FP32 storage, not BF16");
    amr_dev_state_bytes(d, &a->bytes);
    AHIP(a, hipSetDevice(d->c.device));
    AHIP(a, hipMalloc(&a->pool, a->bytes));
@@ -710,7 +723,7 @@ int amr_adj_get_block(amr_adj *a, int slot, double *cells)
       return aerr(a, AMR_ERR_ARG, "get_block: bad arguments");
    amr_dev *d = a->d;
    const int ncell = d->g.gx * d->g.gy * d->g.gz;
-   int rc = by_precision(d->c.precision, [&](auto z) -> int {
+   int rc = adj_precision(d->c.precision, [&](auto z) -> int {
       typedef decltype(z) T;
       hipLaunchKernelGGL(k_adj_get<T>, dim3((ncell + 255) / 256, d->c.num_vars), dim3(256), 0,
                          d->stream, (const T *) a->pool, a->d_stage, d->g, slot);
@@ -733,7 +746,7 @@ int amr_adj_set_block(amr_adj *a, int slot, const double *cells)
    const int ncell = d->g.gx * d->g.gy * d->g.gz;
    AHIP(a, hipMemcpyAsync(a->d_stage, cells, sizeof(double) * ncell * d->c.num_vars,
                           hipMemcpyHostToDevice, d->stream));
-   int rc = by_precision(d->c.precision, [&](auto z) -> int {
+   int rc = adj_precision(d->c.precision, [&](auto z) -> int {
       typedef decltype(z) T;
       hipLaunchKernelGGL(k_adj_set<T>, dim3((ncell + 255) / 256, d->c.num_vars), dim3(256), 0,
                          d->stream, (T *) a->pool, (const double *) a->d_stage, d->g, slot);

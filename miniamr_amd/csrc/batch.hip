@@ -58,8 +58,6 @@ struct BMember {
    Lay L;
    int n_active, num_vars, part_stride, cls;
    int count[3], joff[3];
-   int n_items;   // stencil items: n_active * ceil(num_vars / vpw)
-   int wg;        // 1: stencil items computed by whole workgroups
 };
 
 // Shape classes of the wave-per-item stencil: columns (ny*nz) per lane and
@@ -93,7 +91,7 @@ __device__ __forceinline__ int first_item()
    return __builtin_amdgcn_readfirstlane((int) (blockIdx.x * kWavesPerWg + (threadIdx.x >> 6)));
 }
 
-template <typename T>
+template <typename T, int KV>
 __global__ __launch_bounds__(256) void kb_exchange(const BMember *__restrict__ mem,
                                                    const int *__restrict__ pre, int M, int dir)
 {
@@ -105,63 +103,62 @@ __global__ __launch_bounds__(256) void kb_exchange(const BMember *__restrict__ m
       const int local = t - pre[m], cnt = b.count[dir];
       const int g = local / cnt, j = local - g * cnt;
       const amr_face_job jb = b.jobs[b.joff[dir] + j];
-      const int v0 = g * kVars;
-      exchange_job<T>((T *) b.pool, jb, b.L, v0, min(kVars, b.num_vars - v0), lane, 64);
+      const int v0 = g * KV;
+      exchange_job<T, KV>((T *) b.pool, jb, b.L, v0, min(KV, b.num_vars - v0), lane, 64);
    }
 }
 
-// Stencil list entries are workgroup items: for a member in workgroup mode
-// one (block, variable group) item computed by all 256 threads (the
-// single-instance kernel's shape, for large blocks); in wavefront mode four
-// consecutive items, one per wave (for small blocks, whose planes would
-// leave most of a workgroup idle).
-template <typename T, int ST, int VPW>
+// One stencil kernel per block-size class (the register needs of the
+// classes differ 2.5x, and a kernel holding every class's code would run
+// all of them at the largest one's occupancy).  Each walks the class's
+// items over all members: in wavefront mode one (block, variable group)
+// item per wave, in workgroup mode one per 256-thread workgroup (the
+// single-instance kernel's shape, for large blocks).
+template <int C> struct Cls;
+template <> struct Cls<0> { enum { NC = 1, NP = 2, NZ = 8, D = 8 }; };
+template <> struct Cls<1> { enum { NC = 1, NP = 2, NZ = 0, D = 8 }; };
+template <> struct Cls<2> { enum { NC = 3, NP = 4, NZ = 12, D = 4 }; };
+template <> struct Cls<3> { enum { NC = 3, NP = 4, NZ = 0, D = 4 }; };
+template <> struct Cls<4> { enum { NC = 4, NP = 6, NZ = 16, D = 4 }; };
+template <> struct Cls<5> { enum { NC = 4, NP = 6, NZ = 0, D = 4 }; };
+
+template <typename T, int ST, int VPW, int C, bool WGM>
 __global__ __launch_bounds__(256) void kb_stencil(const BMember *__restrict__ mem,
                                                   const int *__restrict__ pre, int M,
                                                   int want_partials, int ring_elems)
 {
    typedef typename Acc<T>::type A;
+   typedef Cls<C> K;
    extern __shared__ __align__(16) unsigned char smem[];
-   __shared__ double red[kThreads / 64];
-   const int wave = threadIdx.x >> 6;
-   A *wave_ring = (A *) smem + wave * ring_elems;
    const int total = pre[M];
-   for (int t = blockIdx.x; t < total; t += gridDim.x) {
-      const int m = find_member(pre, M, t);
-      const BMember &b = mem[m];
-      __syncthreads();   // every wave is done with the previous item's LDS
-      const int wg = b.wg;
-      const int item = wg ? t - pre[m] : (t - pre[m]) * kWavesPerWg + wave;
-      if (item >= b.n_items)
-         continue;
-      const int na = b.n_active;
-      const int g = item / na, in = item - g * na;
-      const int v0 = g * VPW, nv = min(VPW, b.num_vars - v0);
-      const Lay L = b.L;
-      T *rec = (T *) b.pool + (int64_t) v0 * L.var_stride + (int64_t) b.sorted[in] * L.rec;
-      double *part = want_partials ? b.partials + (int64_t) v0 * b.part_stride + in : nullptr;
-      if (wg) {
-#define WITEM(NZ) \
-   stencil_block<T, ST, 1, 2, VPW, kThreads, NZ>(rec, L, nv, (A *) smem, red, part, b.part_stride)
-         switch (b.cls) {
-         case 0: WITEM(8); break;
-         case 2: WITEM(12); break;
-         case 4: WITEM(16); break;
-         default: WITEM(0); break;
-         }
-#undef WITEM
-      } else {
-#define ITEM(NC, NP, NZ) \
-   stencil_block<T, ST, NC, NP, VPW, 64, NZ>(rec, L, nv, wave_ring, nullptr, part, b.part_stride)
-         switch (b.cls) {
-         case 0: ITEM(1, 2, 8); break;
-         case 1: ITEM(1, 2, 0); break;
-         case 2: ITEM(3, 4, 12); break;
-         case 3: ITEM(3, 4, 0); break;
-         case 4: ITEM(4, 6, 16); break;
-         default: ITEM(4, 6, 0); break;
-         }
-#undef ITEM
+   if constexpr (WGM) {
+      __shared__ double red[kThreads / 64];
+      for (int t = blockIdx.x; t < total; t += gridDim.x) {
+         const int m = find_member(pre, M, t);
+         const BMember &b = mem[m];
+         const int item = t - pre[m], na = b.n_active;
+         const int g = item / na, in = item - g * na;
+         const int v0 = g * VPW, nv = min(VPW, b.num_vars - v0);
+         const Lay L = b.L;
+         T *rec = (T *) b.pool + (int64_t) v0 * L.var_stride + (int64_t) b.sorted[in] * L.rec;
+         double *part = want_partials ? b.partials + (int64_t) v0 * b.part_stride + in : nullptr;
+         stencil_block<T, ST, 1, 2, VPW, kThreads, K::NZ>(rec, L, nv, (A *) smem, red, part,
+                                                          b.part_stride);
+      }
+   } else {
+      A *ring = (A *) smem + (threadIdx.x >> 6) * ring_elems;
+      const int stride = gridDim.x * kWavesPerWg;
+      for (int t = first_item(); t < total; t += stride) {
+         const int m = find_member(pre, M, t);
+         const BMember &b = mem[m];
+         const int item = t - pre[m], na = b.n_active;
+         const int g = item / na, in = item - g * na;
+         const int v0 = g * VPW, nv = min(VPW, b.num_vars - v0);
+         const Lay L = b.L;
+         T *rec = (T *) b.pool + (int64_t) v0 * L.var_stride + (int64_t) b.sorted[in] * L.rec;
+         double *part = want_partials ? b.partials + (int64_t) v0 * b.part_stride + in : nullptr;
+         stencil_block<T, ST, K::NC, K::NP, VPW, 64, K::NZ, K::D>(rec, L, nv, ring, nullptr, part,
+                                                                b.part_stride);
       }
    }
 }
@@ -199,22 +196,24 @@ struct amr_batch {
 
    // work lists
    BMember *d_mem = nullptr;
-   int *d_pre = nullptr;             // ex[3][M+1], st[M+1], rd[M+1]
+   int *d_pre = nullptr;             // ex[3][M+1], st[6][M+1] (per class), rd[M+1]
    int mem_cap = 0;
    int M = 0, plane_max = 0;
-   int tot_ex[3] = {0, 0, 0}, tot_st = 0, tot_rd = 0;
+   int tot_ex[3] = {0, 0, 0}, tot_st[6] = {0, 0, 0, 0, 0, 0}, tot_rd = 0;
+   int plane_max_c[6] = {0, 0, 0, 0, 0, 0};
    bool table_ok = false;
-   int table_vpw = 0, table_wg = -1;
+   int table_vpw = 0, table_wg = -1, table_exv = 0;
    long epoch = 0;
    std::vector<int> member_rows;     // sum_rows of each table member
 
    // launch shape
    int vpw = 1, st_wgs = 0, ex_wgs = 0;
    int wg_mask = 0;                  // bit c: size class c in workgroup mode
+   int ex_vars = kVars;              // variables per exchange item (4, 8 or 16)
 
    // graph
    hipGraphExec_t gexec = nullptr;
-   long gkey[8] = {0};
+   long gkey[9] = {0};
 
    // timing
    bool timing = false;
@@ -283,7 +282,8 @@ int ceil_div(int a, int b) { return (a + b - 1) / b; }
 // a member joined or left, or the stencil's variables per item changed.
 int ensure_table(amr_batch *b)
 {
-   bool need = !b->table_ok || b->table_vpw != b->vpw || b->table_wg != b->wg_mask;
+   bool need = !b->table_ok || b->table_vpw != b->vpw || b->table_wg != b->wg_mask ||
+               b->table_exv != b->ex_vars;
    for (size_t i = 0; i < b->members.size() && !need; i++)
       if (b->seen[i] != b->members[i]->plan_version)
          need = true;
@@ -295,14 +295,20 @@ int ensure_table(amr_batch *b)
       if (b->active[i])
          order.push_back((int) i);
    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-      return shape_class(b->members[x]->g) / 2 > shape_class(b->members[y]->g) / 2;
+      return shape_class(b->members[x]->g) > shape_class(b->members[y]->g);
    });
    const int M = (int) order.size();
+   const int P = M + 1;
    std::vector<BMember> hm(M);
-   std::vector<int> pre(5 * (M + 1), 0);
-   int *ex[3] = {&pre[0], &pre[M + 1], &pre[2 * (M + 1)]};
-   int *st = &pre[3 * (M + 1)], *rd = &pre[4 * (M + 1)];
+   std::vector<int> pre(10 * P, 0);
+   int *ex[3] = {&pre[0], &pre[P], &pre[2 * P]};
+   int *st[6];
+   for (int c = 0; c < 6; c++)
+      st[c] = &pre[(3 + c) * P];
+   int *rd = &pre[9 * P];
    b->plane_max = 0;
+   for (int c = 0; c < 6; c++)
+      b->plane_max_c[c] = 0;
    b->member_rows.assign(M, 0);
    for (int q = 0; q < M; q++) {
       amr_dev *d = b->members[order[q]];
@@ -322,13 +328,15 @@ int ensure_table(amr_batch *b)
          m.count[dir] = d->count[dir];
          m.joff[dir] = off;
          off += d->count[dir];
-         ex[dir][q + 1] = ex[dir][q] + d->count[dir] * ceil_div(m.num_vars, kVars);
+         ex[dir][q + 1] = ex[dir][q] + d->count[dir] * ceil_div(m.num_vars, b->ex_vars);
       }
-      m.n_items = d->n_active * ceil_div(m.num_vars, b->vpw);
-      m.wg = (b->wg_mask >> (m.cls / 2)) & 1;
-      st[q + 1] = st[q] + (m.wg ? m.n_items : ceil_div(m.n_items, kWavesPerWg));
+      const int items = d->n_active * ceil_div(m.num_vars, b->vpw);
+      for (int c = 0; c < 6; c++)
+         st[c][q + 1] = st[c][q] + (c == m.cls ? items : 0);
       rd[q + 1] = rd[q] + m.num_vars;
-      b->plane_max = std::max(b->plane_max, d->g.gy * d->g.gz);
+      const int pl = d->g.gy * d->g.gz;
+      b->plane_max = std::max(b->plane_max, pl);
+      b->plane_max_c[m.cls] = std::max(b->plane_max_c[m.cls], pl);
       b->member_rows[q] = d->d_sums ? d->sum_rows : 0;
    }
    BHIP(b, hipStreamSynchronize(b->stream));   // queued kernels may read the table
@@ -340,24 +348,26 @@ int ensure_table(amr_batch *b)
       b->d_pre = nullptr;
       b->mem_cap = 0;
       BHIP(b, hipMalloc(&b->d_mem, sizeof(BMember) * cap));
-      BHIP(b, hipMalloc(&b->d_pre, sizeof(int) * 5 * (cap + 1)));
+      BHIP(b, hipMalloc(&b->d_pre, sizeof(int) * 10 * (cap + 1)));
       b->mem_cap = cap;
    }
    if (!b->d_pre)
-      BHIP(b, hipMalloc(&b->d_pre, sizeof(int) * 5));
+      BHIP(b, hipMalloc(&b->d_pre, sizeof(int) * 10));
    if (M)
       BHIP(b, hipMemcpy(b->d_mem, hm.data(), sizeof(BMember) * M, hipMemcpyHostToDevice));
    BHIP(b, hipMemcpy(b->d_pre, pre.data(), sizeof(int) * pre.size(), hipMemcpyHostToDevice));
    b->M = M;
    for (int dir = 0; dir < 3; dir++)
       b->tot_ex[dir] = ex[dir][M];
-   b->tot_st = st[M];
+   for (int c = 0; c < 6; c++)
+      b->tot_st[c] = st[c][M];
    b->tot_rd = rd[M];
    for (size_t i = 0; i < b->members.size(); i++)
       b->seen[i] = b->members[i]->plan_version;
    b->table_ok = true;
    b->table_vpw = b->vpw;
    b->table_wg = b->wg_mask;
+   b->table_exv = b->ex_vars;
    b->epoch++;
    return AMR_OK;
 }
@@ -368,18 +378,39 @@ int grid_of(int wgs, int total)
    return wgs > 0 ? std::min(wgs, need) : need;
 }
 
+template <typename T, int ST, int VPW, int C>
+int launch_class(amr_batch *b, int want_partials)
+{
+   typedef typename Acc<T>::type A;
+   const int total = b->tot_st[C];
+   if (!total)
+      return AMR_OK;
+   const int *pre = b->d_pre + (3 + C) * (b->M + 1);
+   if ((b->wg_mask >> (C / 2)) & 1) {
+      const size_t lds = sizeof(A) * 3 * VPW * (size_t) b->plane_max_c[C];
+      const int grid = b->st_wgs > 0 ? std::min(b->st_wgs, total) : total;
+      hipLaunchKernelGGL((kb_stencil<T, ST, VPW, C, true>), dim3(grid), dim3(256), lds,
+                         b->stream, b->d_mem, pre, b->M, want_partials, 0);
+   } else {
+      const int ring = 3 * VPW * b->plane_max_c[C];
+      const size_t lds = sizeof(A) * (size_t) ring * kWavesPerWg;
+      hipLaunchKernelGGL((kb_stencil<T, ST, VPW, C, false>), dim3(grid_of(b->st_wgs, total)),
+                         dim3(256), lds, b->stream, b->d_mem, pre, b->M, want_partials, ring);
+   }
+   BHIP(b, hipGetLastError());
+   return AMR_OK;
+}
+
 template <typename T, int ST, int VPW>
 int launch_stencil_t(amr_batch *b, int want_partials)
 {
-   typedef typename Acc<T>::type A;
-   const int ring = 3 * VPW * b->plane_max;
-   const size_t lds = sizeof(A) * (size_t) ring * kWavesPerWg;
-   const int grid = b->st_wgs > 0 ? std::min(b->st_wgs, b->tot_st) : b->tot_st;
-   hipLaunchKernelGGL((kb_stencil<T, ST, VPW>), dim3(grid), dim3(256),
-                      lds, b->stream, b->d_mem, b->d_pre + 3 * (b->M + 1), b->M, want_partials,
-                      ring);
-   BHIP(b, hipGetLastError());
-   return AMR_OK;
+   int rc = launch_class<T, ST, VPW, 4>(b, want_partials);
+   if (!rc) rc = launch_class<T, ST, VPW, 5>(b, want_partials);
+   if (!rc) rc = launch_class<T, ST, VPW, 2>(b, want_partials);
+   if (!rc) rc = launch_class<T, ST, VPW, 3>(b, want_partials);
+   if (!rc) rc = launch_class<T, ST, VPW, 0>(b, want_partials);
+   if (!rc) rc = launch_class<T, ST, VPW, 1>(b, want_partials);
+   return rc;
 }
 
 template <typename T>
@@ -392,12 +423,16 @@ int enqueue_stage_t(amr_batch *b, int stage, int permute, int row)
       const int dir = permute ? perm[stage % 6][o] : o;
       if (!b->tot_ex[dir])
          continue;
-      hipLaunchKernelGGL(kb_exchange<T>, dim3(grid_of(b->ex_wgs, b->tot_ex[dir])), dim3(256), 0,
+      auto k = b->ex_vars == 16 ? kb_exchange<T, 16>
+             : b->ex_vars == 4 ? kb_exchange<T, 4> : kb_exchange<T, 8>;
+      hipLaunchKernelGGL(k, dim3(grid_of(b->ex_wgs, b->tot_ex[dir])), dim3(256), 0,
                          b->stream, b->d_mem, b->d_pre + dir * (b->M + 1), b->M, dir);
       BHIP(b, hipGetLastError());
    }
    span_end(b);
-   if (b->tot_st) {
+   const int tot_st = b->tot_st[0] + b->tot_st[1] + b->tot_st[2] + b->tot_st[3] +
+                      b->tot_st[4] + b->tot_st[5];
+   if (tot_st) {
       span_begin(b, 1);
       const int wp = row >= 0;
       int rc;
@@ -412,7 +447,7 @@ int enqueue_stage_t(amr_batch *b, int stage, int permute, int row)
    if (row >= 0 && b->tot_rd) {
       span_begin(b, 2);
       hipLaunchKernelGGL(kb_reduce, dim3(grid_of(0, b->tot_rd)), dim3(256), 0, b->stream,
-                         b->d_mem, b->d_pre + 4 * (b->M + 1), b->M, row);
+                         b->d_mem, b->d_pre + 9 * (b->M + 1), b->M, row);
       span_end(b);
       BHIP(b, hipGetLastError());
    }
@@ -583,28 +618,30 @@ int amr_batch_set_active(amr_batch *b, amr_dev *member, int on)
    return berr(b, AMR_ERR_ARG, "set_active: not a member of this batch");
 }
 
-int amr_batch_set_shape(amr_batch *b, int vpw, int stencil_wgs, int exchange_wgs,
-                        int wg_classes)
+int amr_batch_set_shape(amr_batch *b, const amr_batch_shape *sh)
 {
-   if (!b || (vpw != 1 && vpw != 2) || stencil_wgs < 0 || exchange_wgs < 0 ||
-       wg_classes < 0 || wg_classes > 7)
-      return berr(b, AMR_ERR_ARG, "set_shape: vpw must be 1 or 2, grid sizes >= 0, "
-                                  "wg_classes in 0..7");
-   b->vpw = vpw;
-   b->st_wgs = stencil_wgs;
-   b->ex_wgs = exchange_wgs;
-   b->wg_mask = wg_classes;
+   if (!b || !sh || (sh->vpw != 1 && sh->vpw != 2) || sh->stencil_wgs < 0 ||
+       sh->exchange_wgs < 0 || sh->wg_classes < 0 || sh->wg_classes > 7 ||
+       (sh->exchange_vars != 4 && sh->exchange_vars != 8 && sh->exchange_vars != 16))
+      return berr(b, AMR_ERR_ARG, "set_shape: vpw 1 or 2, grid sizes >= 0, wg_classes in "
+                                  "0..7, exchange_vars 4, 8 or 16");
+   b->vpw = sh->vpw;
+   b->st_wgs = sh->stencil_wgs;
+   b->ex_wgs = sh->exchange_wgs;
+   b->wg_mask = sh->wg_classes;
+   b->ex_vars = sh->exchange_vars;
    return AMR_OK;
 }
 
-int amr_batch_get_shape(const amr_batch *b, int shape[4])
+int amr_batch_get_shape(const amr_batch *b, amr_batch_shape *sh)
 {
-   if (!b || !shape)
+   if (!b || !sh)
       return AMR_ERR_ARG;
-   shape[0] = b->vpw;
-   shape[1] = b->st_wgs;
-   shape[2] = b->ex_wgs;
-   shape[3] = b->wg_mask;
+   sh->vpw = b->vpw;
+   sh->stencil_wgs = b->st_wgs;
+   sh->exchange_wgs = b->ex_wgs;
+   sh->wg_classes = b->wg_mask;
+   sh->exchange_vars = b->ex_vars;
    return AMR_OK;
 }
 
@@ -631,8 +668,8 @@ int amr_batch_timestep(amr_batch *b, int stage0, int nstages, int checksum_freq,
       return rc;
    if (checksum_freq && (rc = check_row(b, (nstages - 1) / checksum_freq)))
       return rc;
-   const long key[8] = {b->epoch, permute ? stage0 % 6 : -1, nstages, checksum_freq,
-                        b->vpw, b->st_wgs, b->ex_wgs, (long) b->wg_mask};
+   const long key[9] = {b->epoch, permute ? stage0 % 6 : -1, nstages, checksum_freq,
+                        b->vpw, b->st_wgs, b->ex_wgs, (long) b->wg_mask, (long) b->ex_vars};
    if (!b->gexec || memcmp(key, b->gkey, sizeof(key))) {
       drop_graph(b);
       const bool tm = b->timing;
@@ -672,7 +709,8 @@ int amr_batch_autotune(amr_batch *b, double *ms_out)
    int rc = ensure_table(b);
    if (rc)
       return rc;
-   if (!b->tot_st)
+   if (!(b->tot_st[0] + b->tot_st[1] + b->tot_st[2] + b->tot_st[3] + b->tot_st[4] +
+         b->tot_st[5]))
       return AMR_OK;
    // snapshot the pools of the active members
    std::vector<std::pair<amr_dev *, size_t>> pools;
@@ -697,11 +735,12 @@ int amr_batch_autotune(amr_batch *b, double *ms_out)
    const int reps = 3;
    const int mult[5] = {0, 1, 2, 4, 8};
    double best = 1e30;
-   auto trial = [&](int vpw, int st, int ex, int wgm, bool *better) {
+   auto trial = [&](int vpw, int st, int ex, int wgm, int exv, bool *better) {
       b->vpw = vpw;
       b->st_wgs = st;
       b->ex_wgs = ex;
       b->wg_mask = wgm;
+      b->ex_vars = exv;
       double t = 0.0;
       int r = time_stages(b, reps, &t);
       *better = !r && t < best;
@@ -709,24 +748,28 @@ int amr_batch_autotune(amr_batch *b, double *ms_out)
          best = t;
       return r;
    };
-   int bv = b->vpw, bs = 0, be = 0, bm = 0;
+   int bv = b->vpw, bs = 0, be = 0, bm = 0, bx = kVars;
    bool better = false;
    for (int wgm = 0; wgm < 8 && !rc; wgm++)
-      if (!(rc = trial(bv, bs, be, wgm, &better)) && better)
+      if (!(rc = trial(bv, bs, be, wgm, bx, &better)) && better)
          bm = wgm;
    for (int v = 1; v <= 2 && !rc; v++)
-      if (!(rc = trial(v, bs, be, bm, &better)) && better)
+      if (!(rc = trial(v, bs, be, bm, bx, &better)) && better)
          bv = v;
    for (int si = 1; si < 5 && !rc; si++)
-      if (!(rc = trial(bv, mult[si] * b->cus, be, bm, &better)) && better)
+      if (!(rc = trial(bv, mult[si] * b->cus, be, bm, bx, &better)) && better)
          bs = mult[si] * b->cus;
+   for (int xv : {4, 16})
+      if (!rc && !(rc = trial(bv, bs, be, bm, xv, &better)) && better)
+         bx = xv;
    for (int ei = 1; ei < 5 && !rc; ei++)
-      if (!(rc = trial(bv, bs, mult[ei] * b->cus, bm, &better)) && better)
+      if (!(rc = trial(bv, bs, mult[ei] * b->cus, bm, bx, &better)) && better)
          be = mult[ei] * b->cus;
    b->vpw = bv;
    b->st_wgs = bs;
    b->ex_wgs = be;
    b->wg_mask = bm;
+   b->ex_vars = bx;
    off = 0;
    for (auto &p : pools) {
       hipMemcpyAsync(p.first->pool, backup + off, p.second, hipMemcpyDeviceToDevice, b->stream);

@@ -15,13 +15,13 @@ __device__ __forceinline__ void canon(int dir, int p, int a, int b, int &i, int 
    else { i = a; j = b; k = p; }
 }
 
-constexpr int kVars = 8;   // variables per workgroup, loads issued together
+constexpr int kVars = 8;   // default variables per item, loads issued together
 
 // Fill the ghost plane of face job `jb` for variables v0 .. v0+nv-1
-// (nv <= kVars) of the pool `pool` (the whole [var][slot][record] pool),
+// (nv <= KV) of the pool `pool` (the whole [var][slot][record] pool),
 // with threads tid = 0 .. nthr-1 of the item (a workgroup or a wavefront;
 // no synchronisation is needed: every thread writes its own cells).
-template <typename T>
+template <typename T, int KV = kVars>
 __device__ __forceinline__ void exchange_job(T *__restrict__ pool, const amr_face_job &jb,
                                              const Lay &L, int v0, int nv, int tid, int nthr)
 {
@@ -55,12 +55,12 @@ __device__ __forceinline__ void exchange_job(T *__restrict__ pool, const amr_fac
          const int so = dir == 2 ? zcopy(L, si, sj, sk) : home(L, si, sj, sk);
          const int dh = home(L, i, j, k);
          const int dc = (k == 1 || k == L.nz) ? zcopy(L, i, j, k) : -1;
-         T v[kVars];
+         T v[KV];
 #pragma unroll
-         for (int q = 0; q < kVars; q++)
+         for (int q = 0; q < KV; q++)
             if (q < nv) v[q] = src[q * vs + so];
 #pragma unroll
-         for (int q = 0; q < kVars; q++)
+         for (int q = 0; q < KV; q++)
             if (q < nv) {
                dst[q * vs + dh] = v[q];
                if (dc >= 0) dst[q * vs + dc] = v[q];
@@ -78,12 +78,12 @@ __device__ __forceinline__ void exchange_job(T *__restrict__ pool, const amr_fac
          const int so = dir == 2 ? zcopy(L, si, sj, sk) : home(L, si, sj, sk);
          const int dh = home(L, i, j, k);
          const int dc = (k == 1 || k == L.nz) ? zcopy(L, i, j, k) : -1;
-         T v[kVars];
+         T v[KV];
 #pragma unroll
-         for (int q = 0; q < kVars; q++)
+         for (int q = 0; q < KV; q++)
             if (q < nv) v[q] = src[q * vs + so];
 #pragma unroll
-         for (int q = 0; q < kVars; q++)
+         for (int q = 0; q < KV; q++)
             if (q < nv) {
                const T o = from_acc<T>(to_acc(v[q]) / A(4));
                dst[q * vs + dh] = o;
@@ -106,16 +106,16 @@ __device__ __forceinline__ void exchange_job(T *__restrict__ pool, const amr_fac
          canon(dir, ghost, a + ao, b + bo, i, j, k);
          const int dh = home(L, i, j, k);
          const int dc = (k == 1 || k == L.nz) ? zcopy(L, i, j, k) : -1;
-         T v[kVars][4];
+         T v[KV][4];
 #pragma unroll
-         for (int q = 0; q < kVars; q++)
+         for (int q = 0; q < KV; q++)
             if (q < nv) {
 #pragma unroll
                for (int r = 0; r < 4; r++)
                   v[q][r] = src[q * vs + so[r]];
             }
 #pragma unroll
-         for (int q = 0; q < kVars; q++)
+         for (int q = 0; q < KV; q++)
             if (q < nv) {
                A s = to_acc(v[q][0]);
                s = s + to_acc(v[q][1]);

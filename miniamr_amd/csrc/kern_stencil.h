@@ -91,14 +91,15 @@ __device__ __forceinline__ A divide(A w)
 // variable u at partials[u * part_stride].  Requires ny*nz <= NCOL*NT and
 // (ny+2)*(nz+2) <= NPER*NT.  NZ, when not 0, is the block's nz known at
 // compile time (the LDS neighbour offsets and the column decomposition
-// then become constants); 0 reads it from L.
-template <typename T, int ST, int NCOL, int NPER, int VPW, int NT = kThreads, int NZ = 0>
+// then become constants); 0 reads it from L.  D planes of loads are kept
+// in flight (small blocks can afford all of theirs).
+template <typename T, int ST, int NCOL, int NPER, int VPW, int NT = kThreads, int NZ = 0,
+          int D = kDepth>
 __device__ __forceinline__ void stencil_block(T *__restrict__ rec, const Lay &L, int nv,
                                               typename Acc<T>::type *ring, double *red,
                                               double *__restrict__ partials, int part_stride)
 {
    typedef typename Acc<T>::type A;
-   constexpr int D = kDepth;           // planes of global loads in flight
    const int nz = NZ ? NZ : L.nz, gz = nz + 2;
    const int plane = L.gy * gz, ncol = L.ny * nz, nplanes = L.nx + 2;
    const int64_t vs = L.var_stride;

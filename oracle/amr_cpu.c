@@ -372,7 +372,7 @@ struct amr_batch {
    amr_dev **members;
    char *active;
    int n, cap;
-   int shape[4];
+   amr_batch_shape shape;
    char err[256];
 };
 
@@ -383,7 +383,8 @@ int amr_batch_create(int device, int precision, int stencil, amr_batch **out)
    b->device = device;
    b->precision = precision;
    b->stencil = stencil;
-   b->shape[0] = 1;
+   b->shape.vpw = 1;
+   b->shape.exchange_vars = 8;
    if (precision != AMR_PREC_F64) {
       snprintf(b->err, sizeof(b->err), "the CPU oracle is FP64 only");
       return AMR_ERR_ARG;
@@ -436,19 +437,15 @@ int amr_batch_set_active(amr_batch *b, amr_dev *member, int on)
    return AMR_ERR_ARG;
 }
 
-int amr_batch_set_shape(amr_batch *b, int vpw, int stencil_wgs, int exchange_wgs,
-                        int wg_classes)
+int amr_batch_set_shape(amr_batch *b, const amr_batch_shape *shape)
 {
-   b->shape[0] = vpw;
-   b->shape[1] = stencil_wgs;
-   b->shape[2] = exchange_wgs;
-   b->shape[3] = wg_classes;
+   b->shape = *shape;
    return AMR_OK;
 }
 
-int amr_batch_get_shape(const amr_batch *b, int shape[4])
+int amr_batch_get_shape(const amr_batch *b, amr_batch_shape *shape)
 {
-   memcpy(shape, b->shape, sizeof(b->shape));
+   *shape = b->shape;
    return AMR_OK;
 }
 

@@ -52,8 +52,9 @@ def exes():
     return A.PRODUCT_BATCH_X
 
 
-# launch shapes: vpw, stencil grid, exchange grid, workgroup-mode classes
-SHAPES = {"wave": "1,0,0,0", "wg_all_persistent": "2,512,256,7", "mixed": "1,1024,0,5",
+# launch shapes: vpw, stencil grid, exchange grid, workgroup-mode classes,
+# variables per exchange item
+SHAPES = {"wave": "1,0,0,0,8", "wg_all_persistent": "2,512,256,7,4", "mixed": "1,1024,0,5,16",
           "tuned": None}
 
 
