@@ -21,6 +21,11 @@ cpu_baseline: the reference's OpenMP build (oracle/_ref/miniAMR_omp.x) on
         this host, one MPI rank, on a bounded sample of the same workload
         (its first timestep, --num_tsteps 1), rate from its Summary line.
 
+--workload adjoint times the differentiation form's reverse pass (the
+adjoint kernels over the whole tape of this workload, FP32 storage) next
+to a forward replay of the same tape; the default run reports a summary
+under "differentiation_form".
+
 --workload batch64 instead times the batching form: the 64 independent
 runs of miniamr_amd/configs/batch64.txt (8^3/12^3/16^3 blocks, 8/20/40
 variables, 2-4 refinement levels, FP32 storage) stepped together by
@@ -215,6 +220,95 @@ def batch_summary(b, seq):
         "speedup_vs_sequential": round(v / sv, 3) if sv else None}
 
 
+ADJ_SPHERE = ((0.5, 0.5, 0.5), 0.25)
+
+
+def run_adjoint_workload(precision="f32"):
+    """The reverse pass of the bandwidth-scale workload: one forward run
+    with the tape on, an objective seeded on the blocks that meet a fixed
+    sphere (N(0,1) weights), then the adjoint kernels over the whole tape
+    (all 200 stages and the refinements, last first), timed, next to a
+    forward replay of the same tape."""
+    host, lib = M.load_host_library(), M.load_library()
+    R = M.Run(quiet(CFG1) + " --tape --precision " + precision, host=host)
+    while R.step() == 0:
+        pass
+    st = R.stats()
+    dev = ctypes.c_void_p(R.device())
+    lib.amr_dev_tape(dev, 0)
+    lib.amr_dev_set_timing(dev, 0)
+    n = ctypes.c_int()
+    lib.amr_dev_tape_length(dev, ctypes.byref(n))
+    first = 0
+    op = (ctypes.c_int32 * 8)()
+    while first < n.value and lib.amr_dev_tape_op(dev, first, op) == 0 and op[0] == 1:
+        first += 1
+    adj = ctypes.c_void_p()
+    if lib.amr_adj_create(dev, ctypes.byref(adj)):
+        raise M.AmrError(lib.amr_adj_last_error(adj).decode())
+    import numpy as np
+    rng = np.random.default_rng(7)
+    (cx, cy, cz), r = ADJ_SPHERE
+    seeded = 0
+    for slot, _, lo, hi in R.blocks():
+        near = [min(max(c, lo[d]), hi[d]) for d, c in enumerate((cx, cy, cz))]
+        if sum((near[d] - (cx, cy, cz)[d]) ** 2 for d in range(3)) > r * r:
+            continue
+        w = rng.standard_normal((st.num_vars, st.nx + 2, st.ny + 2, st.nz + 2))
+        lib.amr_adj_set_block(adj, slot, w.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        seeded += 1
+    lib.amr_adj_sync(adj)
+    t0 = time.perf_counter()
+    rc = lib.amr_adj_run(adj, first, n.value)
+    lib.amr_adj_sync(adj)
+    t_adj = time.perf_counter() - t0
+    if rc:
+        raise M.AmrError(lib.amr_adj_last_error(adj).decode())
+    t0 = time.perf_counter()
+    rc = lib.amr_dev_replay(dev, first, n.value)
+    lib.amr_dev_sync(dev)
+    t_fwd = time.perf_counter() - t0
+    lib.amr_adj_destroy(adj)
+    R.finish()
+    R.close()
+    return {"updates": st.cell_var_updates, "adjoint_s": t_adj, "replay_s": t_fwd,
+            "ops": n.value - first, "seeded_blocks": seeded, "precision": precision}
+
+
+def adjoint_summary(a):
+    return {"value": round(a["updates"] / a["adjoint_s"] / 1e9, 4),
+            "unit": "GCell-var updates/s (reverse pass)", "dtype": a["precision"],
+            "forward_replay_value": round(a["updates"] / a["replay_s"] / 1e9, 4),
+            "adjoint_over_forward_time": round(a["adjoint_s"] / a["replay_s"], 3),
+            "adjoint_s": round(a["adjoint_s"], 3), "tape_ops": a["ops"],
+            "seeded_blocks": a["seeded_blocks"], "cell_var_updates": a["updates"]}
+
+
+def main_adjoint(a):
+    r = run_adjoint_workload(a.precision if a.precision != "f64" else "f32")
+    summ = adjoint_summary(r)
+    line = {
+        "metric": "GCell-var updates/s", "value": summ["value"], "unit": "GCell-var updates/s",
+        "n_gpus": 1, "steps": 10, "warmup": 0,
+        "ms_per_step": round(r["adjoint_s"] / 10 * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": summ["dtype"],
+        "data": "synthetic: the bandwidth-scale workload's tape, N(0,1) objective weights",
+        "config": {"workload": "differentiation form: reverse pass (adjoint kernels) over the "
+                               "whole tape of the 4x4x4 x 16^3, 40-variable, 4-level workload",
+                   "tape_ops": summ["tape_ops"], "seeded_blocks": summ["seeded_blocks"],
+                   "cell_var_updates": r["updates"]},
+        "roofline": {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(r["updates"] * 2 * PREC_BYTES[1] / r["adjoint_s"] / 1e9 /
+                                   HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "whole reverse pass (k_adj_stencil, k_adj_exchange, split/merge "
+                               "adjoints), wall time"},
+        "forward_replay": {"value": summ["forward_replay_value"],
+                           "adjoint_over_forward_time": summ["adjoint_over_forward_time"]},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main_batch(a):
     runs = batch_runs()
     b = run_batch_workload(runs)
@@ -269,7 +363,8 @@ def main():
                     choices=["stage", "var"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-precision-report", action="store_true")
-    ap.add_argument("--workload", default="cfg1", choices=["cfg1", "batch64"])
+    ap.add_argument("--workload", default="cfg1", choices=["cfg1", "batch64", "adjoint"])
+    ap.add_argument("--no-adjoint-report", action="store_true")
     ap.add_argument("--no-batch-report", action="store_true")
     ap.add_argument("--no-sequential", action="store_true")
     a = ap.parse_args()
@@ -280,6 +375,9 @@ def main():
         sys.exit(2)
     if a.workload == "batch64":
         main_batch(a)
+        return
+    if a.workload == "adjoint":
+        main_adjoint(a)
         return
     if a.warmup + a.steps > 10 or a.steps < 1:
         print("bench.py: the workload has 10 timesteps; need 1 <= steps and "
@@ -363,6 +461,8 @@ def main():
         "batch_form": None if a.no_batch_report else batch_summary(
             run_batch_workload(batch_runs()),
             None if a.no_sequential else run_sequential(batch_runs())),
+        "differentiation_form": None if a.no_adjoint_report else adjoint_summary(
+            run_adjoint_workload("f32")),
         "cpu_baseline": None if a.no_cpu_baseline else cpu_baseline(),
     }
     print(json.dumps(line), flush=True)

@@ -61,6 +61,8 @@ typedef struct {
                           8^3 blocks; 1: up to 192, e.g. 12^3; 2: up to 256,
                           e.g. 16^3) */
    int exchange_vars;  /* variables per exchange work item: 4, 8 or 16 */
+   int exchange_threads; /* threads per exchange work item: 64 (one wavefront),
+                          128 or 256 (one workgroup) */
 } amr_batch_shape;
 
 int amr_batch_set_shape(amr_batch *b, const amr_batch_shape *shape);

@@ -55,7 +55,7 @@ class BatchShape(ctypes.Structure):
     """amr_batch_shape of include/amr_batch.h."""
     _fields_ = [("vpw", ctypes.c_int), ("stencil_wgs", ctypes.c_int),
                 ("exchange_wgs", ctypes.c_int), ("wg_classes", ctypes.c_int),
-                ("exchange_vars", ctypes.c_int)]
+                ("exchange_vars", ctypes.c_int), ("exchange_threads", ctypes.c_int)]
 
 
 class AmrError(RuntimeError):

@@ -12,11 +12,14 @@
 //   exchange, direction d: (var group g, face job j), local = g*count[d] + j
 //   stencil:               (var group g, sorted block in), local = g*n_active + in
 //   checksum reduce:       variable v, local = v
-// One wavefront computes one item (wave-level synchronisation, its own LDS
-// ring), and the kernels are persistent: the waves of a grid of
-// `wgs` workgroups stride through the list.  Small blocks therefore no
-// longer leave three quarters of a 256-thread workgroup idle, and one
-// launch per stage phase serves every member.
+// Once per rebuild the lists are expanded into flat item records (the job
+// or block, the member, the first variable), so a stage's kernels load one
+// record per item.  Exchange items are computed by a wavefront or a
+// workgroup (64/128/256 threads), stencil items by a wavefront (small
+// blocks, whose planes would leave most of a workgroup idle) or a 256-thread
+// workgroup, one stencil kernel per block-size class; with a grid of `wgs`
+// workgroups the kernels are persistent and stride through the list.  One
+// launch per stage phase (per class for the stencil) serves every member.
 //
 // Arithmetic.  Items run exactly the bodies of the single-instance kernels
 // (kern_stencil.h, kern_exchange.h), so cells are bit-identical to running
@@ -91,20 +94,66 @@ __device__ __forceinline__ int first_item()
    return __builtin_amdgcn_readfirstlane((int) (blockIdx.x * kWavesPerWg + (threadIdx.x >> 6)));
 }
 
-template <typename T, int KV>
-__global__ __launch_bounds__(256) void kb_exchange(const BMember *__restrict__ mem,
-                                                   const int *__restrict__ pre, int M, int dir)
+// Materialised work items.  The prefix tables define the lists; once per
+// rebuild a small kernel expands them into flat records, so that a stage's
+// kernels reach their data with one record load per item instead of a
+// binary search and a chain of dependent descriptor loads.
+struct BExItem {          // exchange: one face job, a group of variables
+   amr_face_job jb;
+   int32_t m, v0;
+};
+struct BStItem {          // stencil: one block, a group of variables
+   int32_t m, slot, v0, in;
+};
+
+__global__ void k_mat_exchange(const BMember *__restrict__ mem, const int *__restrict__ pre,
+                               int M, int dir, int kv, BExItem *__restrict__ out)
 {
-   const int lane = threadIdx.x & 63;
-   const int total = pre[M], stride = gridDim.x * kWavesPerWg;
-   for (int t = first_item(); t < total; t += stride) {
+   const int total = pre[M];
+   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
       const int m = find_member(pre, M, t);
       const BMember &b = mem[m];
       const int local = t - pre[m], cnt = b.count[dir];
       const int g = local / cnt, j = local - g * cnt;
-      const amr_face_job jb = b.jobs[b.joff[dir] + j];
-      const int v0 = g * KV;
-      exchange_job<T, KV>((T *) b.pool, jb, b.L, v0, min(KV, b.num_vars - v0), lane, 64);
+      BExItem it;
+      it.jb = b.jobs[b.joff[dir] + j];
+      it.m = m;
+      it.v0 = g * kv;
+      out[t] = it;
+   }
+}
+
+__global__ void k_mat_stencil(const BMember *__restrict__ mem, const int *__restrict__ pre,
+                              int M, int vpw, BStItem *__restrict__ out)
+{
+   const int total = pre[M];
+   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+      const int m = find_member(pre, M, t);
+      const BMember &b = mem[m];
+      const int local = t - pre[m], na = b.n_active;
+      const int g = local / na, in = local - g * na;
+      BStItem it;
+      it.m = m;
+      it.slot = b.sorted[in];
+      it.v0 = g * vpw;
+      it.in = in;
+      out[t] = it;
+   }
+}
+
+// Exchange items computed by NT threads each: a wavefront (NT = 64, four
+// items per workgroup) or a whole workgroup of NT threads.
+template <typename T, int KV, int NT>
+__global__ __launch_bounds__(NT == 64 ? 256 : NT) void kb_exchange(
+   const BMember *__restrict__ mem, const BExItem *__restrict__ items, int total)
+{
+   const int tid = NT == 64 ? (int) (threadIdx.x & 63) : (int) threadIdx.x;
+   const int per = NT == 64 ? kWavesPerWg : 1;
+   const int first = NT == 64 ? first_item() : (int) blockIdx.x;
+   for (int t = first; t < total; t += gridDim.x * per) {
+      const BExItem it = items[t];
+      const BMember &b = mem[it.m];
+      exchange_job<T, KV>((T *) b.pool, it.jb, b.L, it.v0, min(KV, b.num_vars - it.v0), tid, NT);
    }
 }
 
@@ -124,24 +173,22 @@ template <> struct Cls<5> { enum { NC = 4, NP = 6, NZ = 0, D = 4 }; };
 
 template <typename T, int ST, int VPW, int C, bool WGM>
 __global__ __launch_bounds__(256) void kb_stencil(const BMember *__restrict__ mem,
-                                                  const int *__restrict__ pre, int M,
+                                                  const BStItem *__restrict__ items, int total,
                                                   int want_partials, int ring_elems)
 {
    typedef typename Acc<T>::type A;
    typedef Cls<C> K;
    extern __shared__ __align__(16) unsigned char smem[];
-   const int total = pre[M];
    if constexpr (WGM) {
       __shared__ double red[kThreads / 64];
       for (int t = blockIdx.x; t < total; t += gridDim.x) {
-         const int m = find_member(pre, M, t);
-         const BMember &b = mem[m];
-         const int item = t - pre[m], na = b.n_active;
-         const int g = item / na, in = item - g * na;
-         const int v0 = g * VPW, nv = min(VPW, b.num_vars - v0);
+         const BStItem it = items[t];
+         const BMember &b = mem[it.m];
+         const int nv = min(VPW, b.num_vars - it.v0);
          const Lay L = b.L;
-         T *rec = (T *) b.pool + (int64_t) v0 * L.var_stride + (int64_t) b.sorted[in] * L.rec;
-         double *part = want_partials ? b.partials + (int64_t) v0 * b.part_stride + in : nullptr;
+         T *rec = (T *) b.pool + (int64_t) it.v0 * L.var_stride + (int64_t) it.slot * L.rec;
+         double *part = want_partials ? b.partials + (int64_t) it.v0 * b.part_stride + it.in
+                                      : nullptr;
          stencil_block<T, ST, 1, 2, VPW, kThreads, K::NZ>(rec, L, nv, (A *) smem, red, part,
                                                           b.part_stride);
       }
@@ -149,14 +196,13 @@ __global__ __launch_bounds__(256) void kb_stencil(const BMember *__restrict__ me
       A *ring = (A *) smem + (threadIdx.x >> 6) * ring_elems;
       const int stride = gridDim.x * kWavesPerWg;
       for (int t = first_item(); t < total; t += stride) {
-         const int m = find_member(pre, M, t);
-         const BMember &b = mem[m];
-         const int item = t - pre[m], na = b.n_active;
-         const int g = item / na, in = item - g * na;
-         const int v0 = g * VPW, nv = min(VPW, b.num_vars - v0);
+         const BStItem it = items[t];
+         const BMember &b = mem[it.m];
+         const int nv = min(VPW, b.num_vars - it.v0);
          const Lay L = b.L;
-         T *rec = (T *) b.pool + (int64_t) v0 * L.var_stride + (int64_t) b.sorted[in] * L.rec;
-         double *part = want_partials ? b.partials + (int64_t) v0 * b.part_stride + in : nullptr;
+         T *rec = (T *) b.pool + (int64_t) it.v0 * L.var_stride + (int64_t) it.slot * L.rec;
+         double *part = want_partials ? b.partials + (int64_t) it.v0 * b.part_stride + it.in
+                                      : nullptr;
          stencil_block<T, ST, K::NC, K::NP, VPW, 64, K::NZ, K::D>(rec, L, nv, ring, nullptr, part,
                                                                 b.part_stride);
       }
@@ -210,10 +256,17 @@ struct amr_batch {
    int vpw = 1, st_wgs = 0, ex_wgs = 0;
    int wg_mask = 0;                  // bit c: size class c in workgroup mode
    int ex_vars = kVars;              // variables per exchange item (4, 8 or 16)
+   int ex_threads = 128;             // threads per exchange item (64, 128 or 256)
+
+   // materialised items: exchange per direction, stencil per class
+   BExItem *d_exitems = nullptr;
+   BStItem *d_stitems = nullptr;
+   size_t ex_cap = 0, st_cap = 0;
+   size_t ex_off[3] = {0, 0, 0}, st_off[6] = {0, 0, 0, 0, 0, 0};
 
    // graph
    hipGraphExec_t gexec = nullptr;
-   long gkey[9] = {0};
+   long gkey[10] = {0};
 
    // timing
    bool timing = false;
@@ -362,6 +415,45 @@ int ensure_table(amr_batch *b)
    for (int c = 0; c < 6; c++)
       b->tot_st[c] = st[c][M];
    b->tot_rd = rd[M];
+   // expand the lists into item records
+   size_t nex = 0, nst = 0;
+   for (int dir = 0; dir < 3; dir++) {
+      b->ex_off[dir] = nex;
+      nex += b->tot_ex[dir];
+   }
+   for (int c = 0; c < 6; c++) {
+      b->st_off[c] = nst;
+      nst += b->tot_st[c];
+   }
+   if (nex > b->ex_cap) {
+      hipFree(b->d_exitems);
+      b->d_exitems = nullptr;
+      b->ex_cap = 0;
+      BHIP(b, hipMalloc(&b->d_exitems, sizeof(BExItem) * (nex + nex / 4)));
+      b->ex_cap = nex + nex / 4;
+   }
+   if (nst > b->st_cap) {
+      hipFree(b->d_stitems);
+      b->d_stitems = nullptr;
+      b->st_cap = 0;
+      BHIP(b, hipMalloc(&b->d_stitems, sizeof(BStItem) * (nst + nst / 4)));
+      b->st_cap = nst + nst / 4;
+   }
+   for (int dir = 0; dir < 3 && M; dir++)
+      if (b->tot_ex[dir]) {
+         hipLaunchKernelGGL(k_mat_exchange, dim3(std::min(4096, ceil_div(b->tot_ex[dir], 256))),
+                            dim3(256), 0, b->stream, b->d_mem, b->d_pre + dir * P, M, dir,
+                            b->ex_vars, b->d_exitems + b->ex_off[dir]);
+         BHIP(b, hipGetLastError());
+      }
+   for (int c = 0; c < 6 && M; c++)
+      if (b->tot_st[c]) {
+         hipLaunchKernelGGL(k_mat_stencil, dim3(std::min(4096, ceil_div(b->tot_st[c], 256))),
+                            dim3(256), 0, b->stream, b->d_mem, b->d_pre + (3 + c) * P, M, b->vpw,
+                            b->d_stitems + b->st_off[c]);
+         BHIP(b, hipGetLastError());
+      }
+   BHIP(b, hipStreamSynchronize(b->stream));
    for (size_t i = 0; i < b->members.size(); i++)
       b->seen[i] = b->members[i]->plan_version;
    b->table_ok = true;
@@ -385,17 +477,17 @@ int launch_class(amr_batch *b, int want_partials)
    const int total = b->tot_st[C];
    if (!total)
       return AMR_OK;
-   const int *pre = b->d_pre + (3 + C) * (b->M + 1);
+   const BStItem *items = b->d_stitems + b->st_off[C];
    if ((b->wg_mask >> (C / 2)) & 1) {
       const size_t lds = sizeof(A) * 3 * VPW * (size_t) b->plane_max_c[C];
       const int grid = b->st_wgs > 0 ? std::min(b->st_wgs, total) : total;
       hipLaunchKernelGGL((kb_stencil<T, ST, VPW, C, true>), dim3(grid), dim3(256), lds,
-                         b->stream, b->d_mem, pre, b->M, want_partials, 0);
+                         b->stream, b->d_mem, items, total, want_partials, 0);
    } else {
       const int ring = 3 * VPW * b->plane_max_c[C];
       const size_t lds = sizeof(A) * (size_t) ring * kWavesPerWg;
       hipLaunchKernelGGL((kb_stencil<T, ST, VPW, C, false>), dim3(grid_of(b->st_wgs, total)),
-                         dim3(256), lds, b->stream, b->d_mem, pre, b->M, want_partials, ring);
+                         dim3(256), lds, b->stream, b->d_mem, items, total, want_partials, ring);
    }
    BHIP(b, hipGetLastError());
    return AMR_OK;
@@ -423,10 +515,23 @@ int enqueue_stage_t(amr_batch *b, int stage, int permute, int row)
       const int dir = permute ? perm[stage % 6][o] : o;
       if (!b->tot_ex[dir])
          continue;
-      auto k = b->ex_vars == 16 ? kb_exchange<T, 16>
-             : b->ex_vars == 4 ? kb_exchange<T, 4> : kb_exchange<T, 8>;
-      hipLaunchKernelGGL(k, dim3(grid_of(b->ex_wgs, b->tot_ex[dir])), dim3(256), 0,
-                         b->stream, b->d_mem, b->d_pre + dir * (b->M + 1), b->M, dir);
+      const int total = b->tot_ex[dir];
+      const BExItem *items = b->d_exitems + b->ex_off[dir];
+      if (b->ex_threads == 64) {
+         auto k = b->ex_vars == 16 ? kb_exchange<T, 16, 64>
+                : b->ex_vars == 4 ? kb_exchange<T, 4, 64> : kb_exchange<T, 8, 64>;
+         hipLaunchKernelGGL(k, dim3(grid_of(b->ex_wgs, total)), dim3(256), 0, b->stream,
+                            b->d_mem, items, total);
+      } else {
+         const int grid = b->ex_wgs > 0 ? std::min(b->ex_wgs, total) : total;
+         auto k = b->ex_threads == 256
+                     ? (b->ex_vars == 16 ? kb_exchange<T, 16, 256>
+                        : b->ex_vars == 4 ? kb_exchange<T, 4, 256> : kb_exchange<T, 8, 256>)
+                     : (b->ex_vars == 16 ? kb_exchange<T, 16, 128>
+                        : b->ex_vars == 4 ? kb_exchange<T, 4, 128> : kb_exchange<T, 8, 128>);
+         hipLaunchKernelGGL(k, dim3(grid), dim3(b->ex_threads), 0, b->stream, b->d_mem, items,
+                            total);
+      }
       BHIP(b, hipGetLastError());
    }
    span_end(b);
@@ -558,6 +663,8 @@ int amr_batch_destroy(amr_batch *b)
       hipEventDestroy(e);
    hipFree(b->d_mem);
    hipFree(b->d_pre);
+   hipFree(b->d_exitems);
+   hipFree(b->d_stitems);
    if (b->stream)
       hipStreamDestroy(b->stream);
    delete b;
@@ -622,14 +729,18 @@ int amr_batch_set_shape(amr_batch *b, const amr_batch_shape *sh)
 {
    if (!b || !sh || (sh->vpw != 1 && sh->vpw != 2) || sh->stencil_wgs < 0 ||
        sh->exchange_wgs < 0 || sh->wg_classes < 0 || sh->wg_classes > 7 ||
-       (sh->exchange_vars != 4 && sh->exchange_vars != 8 && sh->exchange_vars != 16))
+       (sh->exchange_vars != 4 && sh->exchange_vars != 8 && sh->exchange_vars != 16) ||
+       (sh->exchange_threads != 64 && sh->exchange_threads != 128 &&
+        sh->exchange_threads != 256))
       return berr(b, AMR_ERR_ARG, "set_shape: vpw 1 or 2, grid sizes >= 0, wg_classes in "
-                                  "0..7, exchange_vars 4, 8 or 16");
+                                  "0..7, exchange_vars 4, 8 or 16, exchange_threads 64, 128 "
+                                  "or 256");
    b->vpw = sh->vpw;
    b->st_wgs = sh->stencil_wgs;
    b->ex_wgs = sh->exchange_wgs;
    b->wg_mask = sh->wg_classes;
    b->ex_vars = sh->exchange_vars;
+   b->ex_threads = sh->exchange_threads;
    return AMR_OK;
 }
 
@@ -642,6 +753,7 @@ int amr_batch_get_shape(const amr_batch *b, amr_batch_shape *sh)
    sh->exchange_wgs = b->ex_wgs;
    sh->wg_classes = b->wg_mask;
    sh->exchange_vars = b->ex_vars;
+   sh->exchange_threads = b->ex_threads;
    return AMR_OK;
 }
 
@@ -668,8 +780,9 @@ int amr_batch_timestep(amr_batch *b, int stage0, int nstages, int checksum_freq,
       return rc;
    if (checksum_freq && (rc = check_row(b, (nstages - 1) / checksum_freq)))
       return rc;
-   const long key[9] = {b->epoch, permute ? stage0 % 6 : -1, nstages, checksum_freq,
-                        b->vpw, b->st_wgs, b->ex_wgs, (long) b->wg_mask, (long) b->ex_vars};
+   const long key[10] = {b->epoch, permute ? stage0 % 6 : -1, nstages, checksum_freq,
+                         b->vpw, b->st_wgs, b->ex_wgs, (long) b->wg_mask, (long) b->ex_vars,
+                         (long) b->ex_threads};
    if (!b->gexec || memcmp(key, b->gkey, sizeof(key))) {
       drop_graph(b);
       const bool tm = b->timing;
@@ -735,12 +848,13 @@ int amr_batch_autotune(amr_batch *b, double *ms_out)
    const int reps = 3;
    const int mult[5] = {0, 1, 2, 4, 8};
    double best = 1e30;
-   auto trial = [&](int vpw, int st, int ex, int wgm, int exv, bool *better) {
+   auto trial = [&](int vpw, int st, int ex, int wgm, int exv, int ext, bool *better) {
       b->vpw = vpw;
       b->st_wgs = st;
       b->ex_wgs = ex;
       b->wg_mask = wgm;
       b->ex_vars = exv;
+      b->ex_threads = ext;
       double t = 0.0;
       int r = time_stages(b, reps, &t);
       *better = !r && t < best;
@@ -748,28 +862,32 @@ int amr_batch_autotune(amr_batch *b, double *ms_out)
          best = t;
       return r;
    };
-   int bv = b->vpw, bs = 0, be = 0, bm = 0, bx = kVars;
+   int bv = b->vpw, bs = 0, be = 0, bm = 0, bx = kVars, bt = 128;
    bool better = false;
    for (int wgm = 0; wgm < 8 && !rc; wgm++)
-      if (!(rc = trial(bv, bs, be, wgm, bx, &better)) && better)
+      if (!(rc = trial(bv, bs, be, wgm, bx, bt, &better)) && better)
          bm = wgm;
    for (int v = 1; v <= 2 && !rc; v++)
-      if (!(rc = trial(v, bs, be, bm, bx, &better)) && better)
+      if (!(rc = trial(v, bs, be, bm, bx, bt, &better)) && better)
          bv = v;
    for (int si = 1; si < 5 && !rc; si++)
-      if (!(rc = trial(bv, mult[si] * b->cus, be, bm, bx, &better)) && better)
+      if (!(rc = trial(bv, mult[si] * b->cus, be, bm, bx, bt, &better)) && better)
          bs = mult[si] * b->cus;
+   for (int xt : {64, 256})
+      if (!rc && !(rc = trial(bv, bs, be, bm, bx, xt, &better)) && better)
+         bt = xt;
    for (int xv : {4, 16})
-      if (!rc && !(rc = trial(bv, bs, be, bm, xv, &better)) && better)
+      if (!rc && !(rc = trial(bv, bs, be, bm, xv, bt, &better)) && better)
          bx = xv;
    for (int ei = 1; ei < 5 && !rc; ei++)
-      if (!(rc = trial(bv, bs, mult[ei] * b->cus, bm, bx, &better)) && better)
+      if (!(rc = trial(bv, bs, mult[ei] * b->cus, bm, bx, bt, &better)) && better)
          be = mult[ei] * b->cus;
    b->vpw = bv;
    b->st_wgs = bs;
    b->ex_wgs = be;
    b->wg_mask = bm;
    b->ex_vars = bx;
+   b->ex_threads = bt;
    off = 0;
    for (auto &p : pools) {
       hipMemcpyAsync(p.first->pool, backup + off, p.second, hipMemcpyDeviceToDevice, b->stream);

@@ -1,11 +1,11 @@
 /*
  * batch_main.c -- miniAMR_batch.x: many independent runs on one GPU.
  *
- *   miniAMR_batch.x [--no_autotune] [--shape vpw,sw,ew,wc,xv] [--quiet] <runs file>
+ *   miniAMR_batch.x [--no_autotune] [--shape vpw,sw,ew,wc,xv,xt] [--quiet] <runs file>
  *
  * --shape fixes the launch shape (amr_batch_shape: variables per stencil
  * item, stencil and exchange grid sizes, workgroup-mode classes, variables
- * per exchange item) instead of autotuning it.
+ * and threads per exchange item) instead of autotuning it.
  *
  * Each non-empty line of the runs file not starting with '#' holds one
  * run's command line in the reference's syntax (the same flags as
@@ -25,15 +25,17 @@
 int main(int argc, char **argv)
 {
    int autotune = 1, quiet = 0, shape_set = 0;
-   amr_batch_shape shp = {1, 0, 0, 0, 8};
+   amr_batch_shape shp = {1, 0, 0, 0, 8, 128};
    const char *path = NULL;
    for (int i = 1; i < argc; i++) {
       if (!strcmp(argv[i], "--no_autotune"))
          autotune = 0;
       else if (!strcmp(argv[i], "--shape") && i + 1 < argc) {
-         if (sscanf(argv[++i], "%d,%d,%d,%d,%d", &shp.vpw, &shp.stencil_wgs, &shp.exchange_wgs,
-                    &shp.wg_classes, &shp.exchange_vars) < 4) {
-            printf("--shape takes vpw,stencil_wgs,exchange_wgs,wg_classes[,exchange_vars]\n");
+         if (sscanf(argv[++i], "%d,%d,%d,%d,%d,%d", &shp.vpw, &shp.stencil_wgs,
+                    &shp.exchange_wgs, &shp.wg_classes, &shp.exchange_vars,
+                    &shp.exchange_threads) < 4) {
+            printf("--shape takes vpw,stencil_wgs,exchange_wgs,wg_classes"
+                   "[,exchange_vars[,exchange_threads]]\n");
             return -1;
          }
          shape_set = 1;
@@ -44,7 +46,7 @@ int main(int argc, char **argv)
          path = argv[i];
    }
    if (!path) {
-      printf("usage: %s [--no_autotune] [--shape vpw,sw,ew,wc,xv] [--quiet] <runs file>\n",
+      printf("usage: %s [--no_autotune] [--shape vpw,sw,ew,wc,xv,xt] [--quiet] <runs file>\n",
              argv[0]);
       return -1;
    }
@@ -116,10 +118,10 @@ int main(int argc, char **argv)
    amr_batch_get_shape(amr_batch_run_batch(B), &sh);
    printf("Batch: runs %d setup %lf time %lf device %lf updates %.6e GCell-var/s %.4lf "
           "shape vpw %d stencil_wgs %d exchange_wgs %d wg_classes %d exchange_vars %d "
-          "tune_ms %.4lf\n",
+          "exchange_threads %d tune_ms %.4lf\n",
           n, t1 - t0, t2 - t1, amr_batch_run_device_ms(B) * 1e-3, updates,
           t2 > t1 ? updates / (t2 - t1) * 1e-9 : 0.0, sh.vpw, sh.stencil_wgs, sh.exchange_wgs,
-          sh.wg_classes, sh.exchange_vars, amr_batch_run_tune_ms(B));
+          sh.wg_classes, sh.exchange_vars, sh.exchange_threads, amr_batch_run_tune_ms(B));
    amr_batch_run_destroy(B);
    return rc == 2;
 }

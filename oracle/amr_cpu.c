@@ -385,6 +385,7 @@ int amr_batch_create(int device, int precision, int stencil, amr_batch **out)
    b->stencil = stencil;
    b->shape.vpw = 1;
    b->shape.exchange_vars = 8;
+   b->shape.exchange_threads = 128;
    if (precision != AMR_PREC_F64) {
       snprintf(b->err, sizeof(b->err), "the CPU oracle is FP64 only");
       return AMR_ERR_ARG;

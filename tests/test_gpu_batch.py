@@ -53,9 +53,9 @@ def exes():
 
 
 # launch shapes: vpw, stencil grid, exchange grid, workgroup-mode classes,
-# variables per exchange item
-SHAPES = {"wave": "1,0,0,0,8", "wg_all_persistent": "2,512,256,7,4", "mixed": "1,1024,0,5,16",
-          "tuned": None}
+# variables and threads per exchange item
+SHAPES = {"wave": "1,0,0,0,8,64", "wg_all_persistent": "2,512,256,7,4,256",
+          "mixed": "1,1024,0,5,16,128", "tuned": None}
 
 
 @pytest.mark.parametrize("common", [COMMON27, COMMON7], ids=["27pt", "7pt_permute"])
