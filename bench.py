@@ -370,7 +370,7 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build runs one rank on one GPU; --gpus %d is not supported "
-              "(the multi-GPU decomposition is a design, see DESIGN.md)" % a.gpus,
+              "(blocks would shard across GPUs as the reference's ranks do; not built)" % a.gpus,
               file=sys.stderr)
         sys.exit(2)
     if a.workload == "batch64":

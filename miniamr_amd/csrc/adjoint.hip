@@ -28,6 +28,7 @@
 #include "amr_adjoint.h"
 #include "amr_internal.h"
 #include "kern_exchange.h"
+#include "kern_stencil.h"
 
 void amr_tape_record(amr_dev *d, int kind, int a1, int a2, int a3, int a4)
 {
@@ -82,65 +83,155 @@ int aerr(amr_adj *a, int rc, const char *fmt, ...)
    } while (0)
 
 // ---------------------------------------------------------------- stencil
-// One workgroup per (active block, variable).  t = mu / DIV on the whole
-// (nx+2)(ny+2)(nz+2) box (zero outside the interior) in LDS; then every
-// cell gathers t over its stencil neighbourhood (the 7/27-point pattern is
-// symmetric): interior cells take the sum (their old values were
-// overwritten), ghost cells add it to their adjoint.
-template <typename T, int ST>
+// One workgroup per (active block, variable), streaming the x planes like
+// the forward kernel.  With t = mu / DIV on the interior and 0 elsewhere
+// (mu = the adjoint of an interior cell: home + z-plane copy), the
+// transposed stencil at any cell c of the (nx+2)(ny+2)(nz+2) box is the sum
+// of t over c's stencil neighbourhood (the 7/27-point pattern is
+// symmetric).  For the 27-point stencil that is S(q-1) + S(q) + S(q+1),
+// S being the 9-term sums of a t plane taken at every (j, k) of the plane,
+// ghost positions included; for the 7-point stencil the centre, its four
+// in-plane and two out-of-plane neighbours.  Interior cells take the value
+// (the forward overwrote them) and their copies clear; ghost cells add it
+// (7-point: face ghosts only, nothing reads edges and corners).
+//
+// Each thread owns plane positions e = tid + q * 256 (q < NP); t planes sit
+// in a 4-deep LDS ring with a zero border, so neighbour reads need no
+// bounds tests; the ghost cells' old adjoints are kept from their load.
+template <typename T, int ST, int NP>
 __global__ __launch_bounds__(256) void k_adj_stencil(T *__restrict__ lam,
                                                       const int32_t *__restrict__ sorted, Lay L,
-                                                      int var)
+                                                      int var0)
 {
    typedef typename Acc<T>::type A;
    extern __shared__ __align__(16) unsigned char smem[];
-   A *tb = (A *) smem;
-   const int gx = L.gx, gy = L.gy, gz = L.gz, box = gx * gy * gz;
-   T *rec = lam + (int64_t) var * L.var_stride + (int64_t) sorted[blockIdx.x] * L.rec;
-   for (int e = threadIdx.x; e < box; e += blockDim.x) {
-      const int i = e / (gy * gz), r = e - i * gy * gz, j = r / gz, k = r - j * gz;
-      A t = 0;
-      if (i >= 1 && i <= L.nx && j >= 1 && j <= L.ny && k >= 1 && k <= L.nz) {
-         A mu = to_acc(rec[home(L, i, j, k)]);
-         if (k == 1 || k == L.nz)
-            mu = mu + to_acc(rec[zcopy(L, i, j, k)]);
-         t = ST == 27 ? mu / A(27) : mu / A(7);
-      }
-      tb[e] = t;
+   const int gy = L.gy, gz = L.gz, plane = gy * gz;
+   const int bw = gz + 2, bplane = (gy + 2) * bw;     // bordered plane
+   A *ring = (A *) smem;                              // 4 bordered planes
+   T *rec = lam + (int64_t) (var0 + blockIdx.y) * L.var_stride +
+            (int64_t) sorted[blockIdx.x] * L.rec;
+   const int tid = threadIdx.x;
+   for (int e = tid; e < 4 * bplane; e += 256)
+      ring[e] = A(0);
+
+   int off0[NP], offN[NP], off1[NP], stp[NP], zc1[NP], bpos[NP];
+   bool inner[NP], live[NP];
+#pragma unroll
+   for (int q = 0; q < NP; q++) {
+      const int e = min(tid + q * 256, plane - 1);
+      const int j = e / gz, k = e - j * gz;
+      live[q] = tid + q * 256 < plane;
+      off0[q] = home(L, 0, j, k);
+      offN[q] = home(L, L.nx + 1, j, k);
+      off1[q] = home(L, 1, j, k);
+      stp[q] = L.nx > 1 ? home(L, 2, j, k) - off1[q] : 0;
+      inner[q] = j >= 1 && j <= L.ny && k >= 1 && k <= L.nz;
+      zc1[q] = inner[q] && (k == 1 || k == L.nz) ? zcopy(L, 1, j, k) : -1;
+      bpos[q] = (j + 1) * bw + (k + 1);
+      if (ST == 7 && !inner[q] && (j == 0 || j == gy - 1) && (k == 0 || k == gz - 1))
+         live[q] = false;   // 7-point: edge lines of interior planes are not touched
    }
+   auto offset = [&](int q, int p) {
+      return p == 0 ? off0[q] : p == L.nx + 1 ? offN[q] : off1[q] + (p - 1) * stp[q];
+   };
+   const int np = L.nx + 2;
+   const A div = ST == 27 ? A(27) : A(7);
+   A s_prev[NP], s_cur[NP], old_prev[NP], old_cur[NP];
+#pragma unroll
+   for (int q = 0; q < NP; q++)
+      s_prev[q] = s_cur[q] = old_prev[q] = old_cur[q] = A(0);
+   // planes of loads in flight: the adjoint of every cell of the plane and,
+   // for interior rows at z = 1 / z = nz, of its copy
+   constexpr int D = kDepth;
+   T pre[D][NP], zpre[D][NP];
+   auto fetch = [&](int d, int p) {
+#pragma unroll
+      for (int q = 0; q < NP; q++)
+         if (tid + q * 256 < plane) {
+            pre[d][q] = rec[offset(q, p)];
+            if (p >= 1 && p <= L.nx && zc1[q] >= 0)
+               zpre[d][q] = rec[zc1[q] + (p - 1) * L.ny];
+         }
+   };
+#pragma unroll
+   for (int d = 0; d < D; d++)
+      if (d < np)
+         fetch(d, d);
    __syncthreads();
-   for (int e = threadIdx.x; e < box; e += blockDim.x) {
-      const int i = e / (gy * gz), r = e - i * gy * gz, j = r / gz, k = r - j * gz;
-      const bool gi = i == 0 || i == gx - 1, gj = j == 0 || j == gy - 1, gk = k == 0 || k == gz - 1;
-      const int nghost = gi + gj + gk;
-      if (ST == 7 && nghost >= 2)
-         continue;   // edges and corners: no 7-point output reads them
-      A s = 0;
-      if (ST == 27) {
-         for (int di = -1; di <= 1; di++)
-            for (int dj = -1; dj <= 1; dj++)
-               for (int dk = -1; dk <= 1; dk++) {
-                  const int a = i + di, b = j + dj, c = k + dk;
-                  if (a >= 0 && a < gx && b >= 0 && b < gy && c >= 0 && c < gz)
-                     s = s + tb[(a * gy + b) * gz + c];
-               }
-      } else {
-         s = tb[e];
-         if (i > 0) s = s + tb[e - gy * gz];
-         if (i < gx - 1) s = s + tb[e + gy * gz];
-         if (j > 0) s = s + tb[e - gz];
-         if (j < gy - 1) s = s + tb[e + gz];
-         if (k > 0) s = s + tb[e - 1];
-         if (k < gz - 1) s = s + tb[e + 1];
+   for (int p0 = 0; p0 <= np; p0 += D) {
+#pragma unroll
+   for (int d = 0; d < D; d++) {
+      const int p = p0 + d;
+      if (p > np)
+         break;
+      // park t(p); keep the old adjoints of plane p for its ghost cells
+      A *slot = ring + (p & 3) * bplane;
+#pragma unroll
+      for (int q = 0; q < NP; q++) {
+         A old = A(0), t = A(0);
+         if (p < np && tid + q * 256 < plane) {
+            old = to_acc(pre[d][q]);
+            if (p >= 1 && p <= L.nx && inner[q]) {
+               A mu = old;
+               if (zc1[q] >= 0)
+                  mu = mu + to_acc(zpre[d][q]);
+               t = mu / div;
+            }
+         }
+         if (tid + q * 256 < plane)
+            slot[bpos[q]] = t;
+         old_prev[q] = old_cur[q];
+         old_cur[q] = old;
       }
-      const int h = home(L, i, j, k);
-      if (nghost == 0) {
-         rec[h] = from_acc<T>(s);
-         if (k == 1 || k == L.nz)
-            rec[zcopy(L, i, j, k)] = from_acc<T>(A(0));
-      } else {
-         rec[h] = from_acc<T>(to_acc(rec[h]) + s);
+      if (p + D < np)
+         fetch(d, p + D);
+      __syncthreads();
+      if (p == 0)
+         continue;
+      const int qo = p - 1;   // output plane
+      const A *tp = ring + (p & 3) * bplane;             // plane p = qo + 1
+      const A *tc = ring + ((p + 3) & 3) * bplane;       // plane qo
+      const A *tm = ring + ((p + 2) & 3) * bplane;       // plane qo - 1 (zeros at qo = 0)
+#pragma unroll
+      for (int q = 0; q < NP; q++) {
+         const int c = bpos[q];
+         A v;
+         if (ST == 27) {
+            A sn = tp[c - bw - 1] + tp[c - bw];
+            sn = sn + tp[c - bw + 1];
+            sn = sn + tp[c - 1];
+            sn = sn + tp[c];
+            sn = sn + tp[c + 1];
+            sn = sn + tp[c + bw - 1];
+            sn = sn + tp[c + bw];
+            sn = sn + tp[c + bw + 1];
+            v = (s_prev[q] + s_cur[q]) + sn;
+            s_prev[q] = s_cur[q];
+            s_cur[q] = sn;
+         } else {
+            v = tm[c] + tc[c - bw];
+            v = v + tc[c - 1];
+            v = v + tc[c];
+            v = v + tc[c + 1];
+            v = v + tc[c + bw];
+            v = v + tp[c];
+            (void) s_prev;
+         }
+         if (!live[q])
+            continue;
+         const bool in_plane = qo >= 1 && qo <= L.nx;
+         if (ST == 7 && !in_plane && !inner[q])
+            continue;   // 7-point: edges and corners of the ghost planes
+         const int o = offset(q, qo);
+         if (in_plane && inner[q]) {
+            rec[o] = from_acc<T>(v);
+            if (zc1[q] >= 0)
+               rec[zc1[q] + (qo - 1) * L.ny] = from_acc<T>(A(0));
+         } else {
+            rec[o] = from_acc<T>(old_prev[q] + v);
+         }
       }
+   }
    }
 }
 
@@ -168,14 +259,12 @@ __device__ __forceinline__ void give(T *rec, int o, typename Acc<T>::type v)
    atomicAdd(&rec[o], v);
 }
 
+// The transpose of one face job's fill for one variable.
 template <typename T>
-__global__ __launch_bounds__(128) void k_adj_exchange(T *__restrict__ lam,
-                                                       const amr_face_job *__restrict__ jobs,
-                                                       Lay L, int var0)
+__device__ __forceinline__ void adj_exchange_job(T *__restrict__ lam, const amr_face_job &jb,
+                                                 const Lay &L, int var)
 {
    typedef typename Acc<T>::type A;
-   const amr_face_job jb = jobs[blockIdx.x];
-   const int var = var0 + blockIdx.y;
    const int dir = jb.face >> 1, hi = jb.face & 1;
    const int n = dir == 0 ? L.nx : dir == 1 ? L.ny : L.nz;
    const int na = dir == 0 ? L.ny : L.nx;
@@ -240,6 +329,18 @@ __global__ __launch_bounds__(128) void k_adj_exchange(T *__restrict__ lam,
          }
       }
    }
+}
+
+// One workgroup per (face job, group of up to 8 variables).
+template <typename T>
+__global__ __launch_bounds__(128) void k_adj_exchange(T *__restrict__ lam,
+                                                       const amr_face_job *__restrict__ jobs,
+                                                       Lay L, int var0, int var_end)
+{
+   const amr_face_job jb = jobs[blockIdx.x];
+   const int v0 = var0 + blockIdx.y * 8, v1 = min(var_end, v0 + 8);
+   for (int var = v0; var < v1; var++)
+      adj_exchange_job<T>(lam, jb, L, var);
 }
 
 // ------------------------------------------------------- split and merge
@@ -387,21 +488,28 @@ int load_plan(amr_adj *a, int plan)
    return AMR_OK;
 }
 
-int adj_stencil(amr_adj *a, int var)
+int adj_stencil(amr_adj *a, int start, int num)
 {
    amr_dev *d = a->d;
-   if (!a->n_active)
+   if (!a->n_active || !num)
       return AMR_OK;
+   const int plane = d->g.gy * d->g.gz;
+   if (plane > 3 * 256)
+      return aerr(a, AMR_ERR_ARG, "adjoint stencil: block face %dx%d too large", d->g.ny, d->g.nz);
    return adj_precision(d->c.precision, [&](auto z) -> int {
       typedef decltype(z) T;
       typedef typename Acc<T>::type A;
-      const size_t lds = sizeof(A) * (size_t) d->g.gx * d->g.gy * d->g.gz;
-      if (d->g.stencil == 27)
-         hipLaunchKernelGGL((k_adj_stencil<T, 27>), dim3(a->n_active), dim3(256), lds, d->stream,
-                            (T *) a->pool, a->d_sorted, d->g, var);
-      else
-         hipLaunchKernelGGL((k_adj_stencil<T, 7>), dim3(a->n_active), dim3(256), lds, d->stream,
-                            (T *) a->pool, a->d_sorted, d->g, var);
+      const size_t lds = sizeof(A) * 4 * (size_t) (d->g.gy + 2) * (d->g.gz + 2);
+      dim3 grid(a->n_active, num);
+#define GO(ST, NP) \
+   hipLaunchKernelGGL((k_adj_stencil<T, ST, NP>), grid, dim3(256), lds, d->stream, \
+                      (T *) a->pool, a->d_sorted, d->g, start)
+      if (d->g.stencil == 27) {
+         if (plane <= 256) GO(27, 1); else if (plane <= 512) GO(27, 2); else GO(27, 3);
+      } else {
+         if (plane <= 256) GO(7, 1); else if (plane <= 512) GO(7, 2); else GO(7, 3);
+      }
+#undef GO
       AHIP(a, hipGetLastError());
       return AMR_OK;
    });
@@ -420,8 +528,8 @@ int adj_comm(amr_adj *a, int start, int num, int stage, int permute)
          continue;
       int rc = adj_precision(d->c.precision, [&](auto z) -> int {
          typedef decltype(z) T;
-         hipLaunchKernelGGL(k_adj_exchange<T>, dim3(n, num), dim3(128), 0, d->stream,
-                            (T *) a->pool, a->d_jobs + off, d->g, start);
+         hipLaunchKernelGGL(k_adj_exchange<T>, dim3(n, (num + 7) / 8), dim3(128), 0, d->stream,
+                            (T *) a->pool, a->d_jobs + off, d->g, start, start + num);
          AHIP(a, hipGetLastError());
          return AMR_OK;
       });
@@ -781,11 +889,10 @@ int amr_adj_run(amr_adj *a, int op0, int op1)
          rc = adj_comm(a, op[1], op[2], op[3], op[4]);
          break;
       case AMR_OP_STENCIL:
-         rc = adj_stencil(a, op[1]);
+         rc = adj_stencil(a, op[1], 1);
          break;
       case AMR_OP_STAGE:
-         for (int v = op[1] + op[2] - 1; v >= op[1] && !rc; v--)
-            rc = adj_stencil(a, v);
+         rc = adj_stencil(a, op[1], op[2]);
          if (!rc)
             rc = adj_comm(a, op[1], op[2], op[3], op[4]);
          break;

@@ -1,7 +1,7 @@
 // exchange.hip -- ghost-plane fills of one exchange direction.
 //
 // One launch per direction phase; one workgroup per (face job, group of
-// up to kVars variables).
+// up to 8 variables).
 // Within a phase every fill is a gather from cells no fill of that phase
 // writes (face-interior planes, and for the 27-point stencil the ghosts of
 // directions done earlier), and no two fills write the same cell, so the
@@ -21,20 +21,58 @@
 // layout (amr_internal.h); the z phase reads source planes z = 1 / z = nz
 // from their contiguous copies.  A workgroup fills one face job for up to
 // kVars variables, issuing all of their loads before the stores.
+#include <stdio.h>
+#include <stdlib.h>
+
 #include "kern_exchange.h"
 
 namespace {
 
 using namespace amrk;
 
-template <typename T>
-__global__ __launch_bounds__(128) void k_exchange(T *__restrict__ pool,
-                                                  const amr_face_job *__restrict__ jobs,
-                                                  Lay L, int var0, int var_end)
+template <typename T, int KV, int NT>
+__global__ __launch_bounds__(NT) void k_exchange(T *__restrict__ pool,
+                                                 const amr_face_job *__restrict__ jobs,
+                                                 Lay L, int var0, int var_end)
 {
    const amr_face_job jb = jobs[blockIdx.x];
-   const int v0 = var0 + blockIdx.y * kVars;
-   exchange_job<T>(pool, jb, L, v0, min(kVars, var_end - v0), threadIdx.x, blockDim.x);
+   const int v0 = var0 + blockIdx.y * KV;
+   exchange_job<T, KV>(pool, jb, L, v0, min(KV, var_end - v0), threadIdx.x, NT);
+}
+
+// Variables and threads per workgroup.  Default 8 x 256 (measured best for
+// FP64 and FP32 on the 16^3 workload: 8 x 128 was 3% slower in FP32, 16
+// variables 1-3% slower); MINIAMR_EXCHANGE ("vars,threads" with vars
+// 4/8/16 and threads 128/256) overrides it.
+struct ExShape {
+   int kv = kVars, nt = 256;
+};
+
+ExShape ex_shape()
+{
+   static ExShape sh;
+   static bool read = false;
+   if (!read) {
+      read = true;
+      const char *e = getenv("MINIAMR_EXCHANGE");
+      int kv = 0, nt = 0;
+      if (e && sscanf(e, "%d,%d", &kv, &nt) == 2 && (kv == 4 || kv == 8 || kv == 16) &&
+          (nt == 128 || nt == 256)) {
+         sh.kv = kv;
+         sh.nt = nt;
+      }
+   }
+   return sh;
+}
+
+template <typename T, int KV, int NT>
+int launch_kv(amr_dev *d, int off, int n, int start, int num)
+{
+   dim3 grid(n, (num + KV - 1) / KV);
+   hipLaunchKernelGGL((k_exchange<T, KV, NT>), grid, dim3(NT), 0, d->stream, (T *) d->pool,
+                      d->d_jobs + off, d->g, start, start + num);
+   HIP_TRY(d, hipGetLastError());
+   return AMR_OK;
 }
 
 template <typename T>
@@ -44,11 +82,14 @@ int launch(amr_dev *d, int dir, int start, int num)
    int n = d->count[dir];
    if (!n)
       return AMR_OK;
-   dim3 grid(n, (num + kVars - 1) / kVars);
-   hipLaunchKernelGGL(k_exchange<T>, grid, dim3(128), 0, d->stream, (T *) d->pool,
-                      d->d_jobs + off, d->g, start, start + num);
-   HIP_TRY(d, hipGetLastError());
-   return AMR_OK;
+   const ExShape sh = ex_shape();
+   if (sh.nt == 256)
+      return sh.kv == 16 ? launch_kv<T, 16, 256>(d, off, n, start, num)
+           : sh.kv == 4 ? launch_kv<T, 4, 256>(d, off, n, start, num)
+                        : launch_kv<T, 8, 256>(d, off, n, start, num);
+   return sh.kv == 16 ? launch_kv<T, 16, 128>(d, off, n, start, num)
+        : sh.kv == 4 ? launch_kv<T, 4, 128>(d, off, n, start, num)
+                     : launch_kv<T, 8, 128>(d, off, n, start, num);
 }
 
 }  // namespace

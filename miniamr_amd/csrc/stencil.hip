@@ -32,7 +32,7 @@ namespace {
 
 using namespace amrk;
 
-template <typename T, int ST, int NCOL, int NPER, int VPW, int NZ>
+template <typename T, int ST, int NCOL, int NPER, int VPW, int NZ, int D = kDepth>
 __global__ __launch_bounds__(kThreads) void k_stencil(T *__restrict__ pool,
                                                        const int32_t *__restrict__ sorted,
                                                        Lay L, int var0, int var_end,
@@ -45,7 +45,7 @@ __global__ __launch_bounds__(kThreads) void k_stencil(T *__restrict__ pool,
    const int in = blockIdx.x, v0 = var0 + blockIdx.y * VPW;
    const int nv = min(VPW, var_end - v0);
    T *rec = pool + (int64_t) v0 * L.var_stride + (int64_t) sorted[in] * L.rec;
-   stencil_block<T, ST, NCOL, NPER, VPW, kThreads, NZ>(rec, L, nv, (A *) smem, red,
+   stencil_block<T, ST, NCOL, NPER, VPW, kThreads, NZ, D>(rec, L, nv, (A *) smem, red,
                                          partials ? partials + (int64_t) v0 * part_stride + in
                                                   : nullptr,
                                          part_stride);
@@ -116,15 +116,37 @@ __global__ void k_convert_out(const T *__restrict__ pool, double *__restrict__ s
 }
 
 // Variables per workgroup: two for the narrow storage types (the per-cell
-// index work and barriers are shared), one for FP64.
+// index work and barriers are shared), one for FP64.  MINIAMR_STENCIL_VPW=4
+// selects four for the narrow types.
 template <typename T> struct Vpw { static constexpr int v = 2; };
 template <> struct Vpw<double> { static constexpr int v = 1; };
 
-template <typename T, int ST>
-int launch_st(amr_dev *d, int start, int num, double *parts)
+static int narrow_vpw()
+{
+   static int v = -1;
+   if (v < 0) {
+      const char *e = getenv("MINIAMR_STENCIL_VPW");
+      v = e ? atoi(e) : 2;
+      if (v != 1 && v != 4)
+         v = 2;
+   }
+   return v;
+}
+
+static int depth6()
+{
+   static int v = -1;
+   if (v < 0) {
+      const char *e = getenv("MINIAMR_STENCIL_DEPTH");
+      v = e && atoi(e) == 6;
+   }
+   return v;
+}
+
+template <typename T, int ST, int V>
+int launch_st_v(amr_dev *d, int start, int num, double *parts)
 {
    const Lay &L = d->g;
-   constexpr int V = Vpw<T>::v;
    const int ncol = L.ny * L.nz, plane = L.gy * L.gz;
    dim3 grid(d->n_active, (num + V - 1) / V);
    size_t lds = sizeof(typename Acc<T>::type) * 3 * V * (size_t) plane;
@@ -134,7 +156,11 @@ int launch_st(amr_dev *d, int start, int num, double *parts)
                       d->c.max_blocks)
    // common face widths get a compile-time nz (constant LDS offsets)
    if (ncol <= kThreads && plane <= 2 * kThreads) {
-      if (L.nz == 16) GO(1, 2, 16);
+      if (L.nz == 16 && depth6())
+         hipLaunchKernelGGL((k_stencil<T, ST, 1, 2, V, 16, 6>), grid, dim3(kThreads), lds,
+                            d->stream, (T *) d->pool, d->d_sorted, L, start, start + num, parts,
+                            d->c.max_blocks);
+      else if (L.nz == 16) GO(1, 2, 16);
       else if (L.nz == 12) GO(1, 2, 12);
       else if (L.nz == 8) GO(1, 2, 8);
       else GO(1, 2, 0);
@@ -148,6 +174,18 @@ int launch_st(amr_dev *d, int start, int num, double *parts)
 #undef GO
    HIP_TRY(d, hipGetLastError());
    return AMR_OK;
+}
+
+template <typename T, int ST>
+int launch_st(amr_dev *d, int start, int num, double *parts)
+{
+   if (Vpw<T>::v == 1)
+      return launch_st_v<T, ST, 1>(d, start, num, parts);
+   if (narrow_vpw() == 4)
+      return launch_st_v<T, ST, 4>(d, start, num, parts);
+   if (narrow_vpw() == 1)
+      return launch_st_v<T, ST, 1>(d, start, num, parts);
+   return launch_st_v<T, ST, 2>(d, start, num, parts);
 }
 
 template <typename T>
