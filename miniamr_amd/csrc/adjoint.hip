@@ -23,6 +23,7 @@
 #include <stdio.h>
 #include <string.h>
 #include <algorithm>
+#include <map>
 #include <set>
 
 #include "amr_adjoint.h"
@@ -53,6 +54,7 @@ struct amr_adj {
    int jobs_cap = 0, slots_cap = 0;
    int plan = -2, n_active = 0;
    int32_t count[3] = {0, 0, 0};
+   bool exclusive[3] = {false, false, false};   // no element read by two fills
    void *d_jobbuf = nullptr;        // split/merge jobs of one reverse segment
    size_t jobbuf_bytes = 0;
    double *d_stage = nullptr;       // one slot, canonical FP64
@@ -250,19 +252,27 @@ __device__ __forceinline__ typename Acc<T>::type take(T *rec, int h, int c)
    return mu;
 }
 
-// Accumulate into a source element.  Atomic: a real mesh never lets two
-// fills of one phase read the same element, but the adjoint does not rely
-// on it (any plan the forward accepts is transposed correctly).
-template <typename T>
+// Accumulate into a source element.  A real mesh never lets two fills of
+// one phase read the same element; the host checks that per plan and
+// direction (exclusive_reads) and otherwise uses atomic adds, so any plan
+// the forward accepts is transposed correctly.
+template <typename T, bool ATOMIC>
 __device__ __forceinline__ void give(T *rec, int o, typename Acc<T>::type v)
 {
-   atomicAdd(&rec[o], v);
+   if (ATOMIC)
+      atomicAdd(&rec[o], v);
+   else
+      rec[o] = from_acc<T>(to_acc(rec[o]) + v);
 }
 
-// The transpose of one face job's fill for one variable.
-template <typename T>
+// The transpose of one face job's fill for variables v0 .. v0+nv-1
+// (nv <= KV), threads tid of nthr: every thread issues all its loads (the
+// ghost adjoints and, without atomics, the source adjoints) for the KV
+// variables before any store, as the forward fill does.
+template <typename T, int KV, bool ATOMIC>
 __device__ __forceinline__ void adj_exchange_job(T *__restrict__ lam, const amr_face_job &jb,
-                                                 const Lay &L, int var)
+                                                 const Lay &L, int v0, int nv, int tid,
+                                                 int nthr)
 {
    typedef typename Acc<T>::type A;
    const int dir = jb.face >> 1, hi = jb.face & 1;
@@ -271,14 +281,16 @@ __device__ __forceinline__ void adj_exchange_job(T *__restrict__ lam, const amr_
    const int nb = dir == 2 ? L.ny : L.nz;
    const int ghost = hi ? n + 1 : 0;
    const int plane = hi ? 1 : n;
-   T *dst = lam + (int64_t) var * L.var_stride + (int64_t) jb.dst * L.rec;
-   T *src = lam + (int64_t) var * L.var_stride + (int64_t) jb.src * L.rec;
+   const int64_t vs = L.var_stride;
+   T *dst = lam + (int64_t) v0 * vs + (int64_t) jb.dst * L.rec;
+   T *src = lam + (int64_t) v0 * vs + (int64_t) jb.src * L.rec;
    auto srcoff = [&](int si, int sj, int sk) {
       return dir == 2 ? zcopy(L, si, sj, sk) : home(L, si, sj, sk);
    };
    auto dstcopy = [&](int i, int j, int k) {
       return (k == 1 || k == L.nz) ? zcopy(L, i, j, k) : -1;
    };
+   const T zero = from_acc<T>(A(0));
    if (jb.kind == AMR_FACE_SAME || jb.kind == AMR_FACE_BC) {
       int ext_a = 0, ext_b = 0;
       if (L.stencil == 27) {
@@ -289,58 +301,122 @@ __device__ __forceinline__ void adj_exchange_job(T *__restrict__ lam, const amr_
       const int a0 = 1 - ext_a, b0 = 1 - ext_b;
       const int wa = na + 2 * ext_a, wb = nb + 2 * ext_b;
       const int sp = jb.kind == AMR_FACE_BC ? (hi ? n : 1) : plane;
-      for (int t = threadIdx.x; t < wa * wb; t += blockDim.x) {
+      for (int t = tid; t < wa * wb; t += nthr) {
          const int a = a0 + t / wb, b = b0 + t % wb;
          int i, j, k, si, sj, sk;
          canon(dir, sp, a, b, si, sj, sk);
          canon(dir, ghost, a, b, i, j, k);
-         const A mu = take(dst, home(L, i, j, k), dstcopy(i, j, k));
-         give(src, srcoff(si, sj, sk), mu);
+         const int so = srcoff(si, sj, sk), dh = home(L, i, j, k), dc = dstcopy(i, j, k);
+         A mu[KV], sv[KV];
+#pragma unroll
+         for (int q = 0; q < KV; q++)
+            if (q < nv) {
+               mu[q] = to_acc(dst[q * vs + dh]);
+               if (dc >= 0) mu[q] = mu[q] + to_acc(dst[q * vs + dc]);
+               if (!ATOMIC) sv[q] = to_acc(src[q * vs + so]);
+            }
+#pragma unroll
+         for (int q = 0; q < KV; q++)
+            if (q < nv) {
+               dst[q * vs + dh] = zero;
+               if (dc >= 0) dst[q * vs + dc] = zero;
+               if (ATOMIC) atomicAdd(&src[q * vs + so], mu[q]);
+               else src[q * vs + so] = from_acc<T>(sv[q] + mu[q]);
+            }
       }
    } else if (jb.kind == AMR_FACE_PROLONG) {
       // each coarse cell of the quarter fed 4 fine ghosts with value / 4
       const int ha = na / 2, hb = nb / 2;
       const int ao = jb.jq * ha, bo = jb.iq * hb;
-      for (int t = threadIdx.x; t < ha * hb; t += blockDim.x) {
+      for (int t = tid; t < ha * hb; t += nthr) {
          const int ca = 1 + t / hb, cb = 1 + t % hb;
-         A mu = 0;
+         int dh[4], dc[4];
+#pragma unroll
          for (int r = 0; r < 4; r++) {
-            const int a = 2 * ca - 1 + (r >> 1), b = 2 * cb - 1 + (r & 1);
             int i, j, k;
-            canon(dir, ghost, a, b, i, j, k);
-            mu = mu + take(dst, home(L, i, j, k), dstcopy(i, j, k));
+            canon(dir, ghost, 2 * ca - 1 + (r >> 1), 2 * cb - 1 + (r & 1), i, j, k);
+            dh[r] = home(L, i, j, k);
+            dc[r] = dstcopy(i, j, k);
          }
          int si, sj, sk;
          canon(dir, plane, ca + ao, cb + bo, si, sj, sk);
-         give(src, srcoff(si, sj, sk), mu / A(4));
+         const int so = srcoff(si, sj, sk);
+         A mu[KV], sv[KV];
+#pragma unroll
+         for (int q = 0; q < KV; q++)
+            if (q < nv) {
+               A m = 0;
+#pragma unroll
+               for (int r = 0; r < 4; r++) {
+                  m = m + to_acc(dst[q * vs + dh[r]]);
+                  if (dc[r] >= 0) m = m + to_acc(dst[q * vs + dc[r]]);
+               }
+               mu[q] = m / A(4);
+               if (!ATOMIC) sv[q] = to_acc(src[q * vs + so]);
+            }
+#pragma unroll
+         for (int q = 0; q < KV; q++)
+            if (q < nv) {
+#pragma unroll
+               for (int r = 0; r < 4; r++) {
+                  dst[q * vs + dh[r]] = zero;
+                  if (dc[r] >= 0) dst[q * vs + dc[r]] = zero;
+               }
+               if (ATOMIC) atomicAdd(&src[q * vs + so], mu[q]);
+               else src[q * vs + so] = from_acc<T>(sv[q] + mu[q]);
+            }
       }
    } else {   // RESTRICT: each coarse ghost was the sum of 4 fine cells
       const int ha = na / 2, hb = nb / 2;
       const int ao = jb.jq * ha, bo = jb.iq * hb;
-      for (int t = threadIdx.x; t < ha * hb; t += blockDim.x) {
+      for (int t = tid; t < ha * hb; t += nthr) {
          const int a = 1 + t / hb, b = 1 + t % hb;
          int i, j, k;
          canon(dir, ghost, a + ao, b + bo, i, j, k);
-         const A mu = take(dst, home(L, i, j, k), dstcopy(i, j, k));
+         const int dh = home(L, i, j, k), dc = dstcopy(i, j, k);
+         int so[4];
+#pragma unroll
          for (int r = 0; r < 4; r++) {
             int si, sj, sk;
             canon(dir, plane, 2 * a - 1 + (r >> 1), 2 * b - 1 + (r & 1), si, sj, sk);
-            give(src, srcoff(si, sj, sk), mu);
+            so[r] = srcoff(si, sj, sk);
          }
+         A mu[KV], sv[KV][4];
+#pragma unroll
+         for (int q = 0; q < KV; q++)
+            if (q < nv) {
+               mu[q] = to_acc(dst[q * vs + dh]);
+               if (dc >= 0) mu[q] = mu[q] + to_acc(dst[q * vs + dc]);
+               if (!ATOMIC) {
+#pragma unroll
+                  for (int r = 0; r < 4; r++)
+                     sv[q][r] = to_acc(src[q * vs + so[r]]);
+               }
+            }
+#pragma unroll
+         for (int q = 0; q < KV; q++)
+            if (q < nv) {
+               dst[q * vs + dh] = zero;
+               if (dc >= 0) dst[q * vs + dc] = zero;
+#pragma unroll
+               for (int r = 0; r < 4; r++) {
+                  if (ATOMIC) atomicAdd(&src[q * vs + so[r]], mu[q]);
+                  else src[q * vs + so[r]] = from_acc<T>(sv[q][r] + mu[q]);
+               }
+            }
       }
    }
 }
 
 // One workgroup per (face job, group of up to 8 variables).
-template <typename T>
-__global__ __launch_bounds__(128) void k_adj_exchange(T *__restrict__ lam,
+template <typename T, bool ATOMIC>
+__global__ __launch_bounds__(256) void k_adj_exchange(T *__restrict__ lam,
                                                        const amr_face_job *__restrict__ jobs,
                                                        Lay L, int var0, int var_end)
 {
    const amr_face_job jb = jobs[blockIdx.x];
-   const int v0 = var0 + blockIdx.y * 8, v1 = min(var_end, v0 + 8);
-   for (int var = v0; var < v1; var++)
-      adj_exchange_job<T>(lam, jb, L, var);
+   const int v0 = var0 + blockIdx.y * 8;
+   adj_exchange_job<T, 8, ATOMIC>(lam, jb, L, v0, min(8, var_end - v0), threadIdx.x, 256);
 }
 
 // ------------------------------------------------------- split and merge
@@ -366,7 +442,7 @@ __global__ void k_adj_split(T *__restrict__ lam, const amr_split_job *__restrict
                    ck = 2 * k + 1 + (q >> 2);
          mu = mu + take(c, home(L, ci, cj, ck), (ck == 1 || ck == L.nz) ? zcopy(L, ci, cj, ck) : -1);
       }
-      give(p, home(L, i + 1 + i1, j + 1 + j1, k + 1 + k1), mu / A(8));
+      give<T, false>(p, home(L, i + 1 + i1, j + 1 + j1, k + 1 + k1), mu / A(8));
    }
 }
 
@@ -388,8 +464,8 @@ __global__ void k_adj_merge(T *__restrict__ lam, const amr_merge_job *__restrict
       const int pi = i + 1 + i1, pj = j + 1 + j1, pk = k + 1 + k1;
       const A mu = take(p, home(L, pi, pj, pk), (pk == 1 || pk == L.nz) ? zcopy(L, pi, pj, pk) : -1);
       for (int q = 0; q < 8; q++)
-         give(c, home(L, 2 * i + 1 + (q & 1), 2 * j + 1 + ((q >> 1) & 1), 2 * k + 1 + (q >> 2)),
-              mu);
+         give<T, false>(c, home(L, 2 * i + 1 + (q & 1), 2 * j + 1 + ((q >> 1) & 1),
+                                2 * k + 1 + (q >> 2)), mu);
    }
 }
 
@@ -450,6 +526,41 @@ int adj_precision(int prec, F f)
    return prec == AMR_PREC_F64 ? f((double) 0) : f((float) 0);
 }
 
+// Per direction: do the plan's fills read disjoint source cells?  Every
+// fill reads one plane of its source block -- plane 1 or n of the
+// direction -- whole (same level, boundary, restriction, with the 27-point
+// extended ranges staying in that plane) or one quarter (prolongation), so
+// they are disjoint unless two fills read the same plane of the same block
+// and one of them reads all of it or both the same quarter.
+void exclusive_reads(amr_adj *a, const TapePlan &pl)
+{
+   int off = 0;
+   for (int dir = 0; dir < 3; dir++) {
+      std::map<std::pair<int, int>, int> seen;   // (slot, plane side) -> quarter mask, 16 = whole
+      bool ok = true;
+      for (int t = off; t < off + pl.count[dir] && ok; t++) {
+         const amr_face_job &jb = pl.jobs[t];
+         const int hi = jb.face & 1;
+         int slot, side, mask;
+         if (jb.kind == AMR_FACE_BC) {
+            slot = jb.dst;
+            side = hi;        // the block's own plane next to the face
+            mask = 16;
+         } else {
+            slot = jb.src;
+            side = !hi;       // the neighbour's plane next to the shared face
+            mask = jb.kind == AMR_FACE_PROLONG ? 1 << (jb.iq * 2 + jb.jq) : 16;
+         }
+         int &m = seen[{slot, side}];
+         if ((m & mask) || (m && mask == 16) || (m & 16))
+            ok = false;
+         m |= mask;
+      }
+      a->exclusive[dir] = ok;
+      off += pl.count[dir];
+   }
+}
+
 int load_plan(amr_adj *a, int plan)
 {
    if (plan == a->plan)
@@ -485,6 +596,7 @@ int load_plan(amr_adj *a, int plan)
    a->n_active = na;
    for (int i = 0; i < 3; i++)
       a->count[i] = pl.count[i];
+   exclusive_reads(a, pl);
    return AMR_OK;
 }
 
@@ -528,7 +640,8 @@ int adj_comm(amr_adj *a, int start, int num, int stage, int permute)
          continue;
       int rc = adj_precision(d->c.precision, [&](auto z) -> int {
          typedef decltype(z) T;
-         hipLaunchKernelGGL(k_adj_exchange<T>, dim3(n, (num + 7) / 8), dim3(128), 0, d->stream,
+         auto k = a->exclusive[dir] ? k_adj_exchange<T, false> : k_adj_exchange<T, true>;
+         hipLaunchKernelGGL(k, dim3(n, (num + 7) / 8), dim3(256), 0, d->stream,
                             (T *) a->pool, a->d_jobs + off, d->g, start, start + num);
          AHIP(a, hipGetLastError());
          return AMR_OK;
