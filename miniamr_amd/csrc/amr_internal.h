@@ -122,6 +122,17 @@ template <> __device__ __forceinline__ uint16_t from_acc<uint16_t>(float v)
    return (uint16_t) (u >> 16);
 }
 
+// Global-memory view of a device pointer.  A pointer that a kernel reads
+// from memory (the batch's item records) is otherwise a generic address,
+// accessed with flat instructions, which count against the LDS wait counter
+// as well: every LDS wait would then also wait for the outstanding global
+// loads.  Only for pointers into device allocations.
+template <typename T> using gptr = __attribute__((address_space(1))) T *;
+template <typename T> __device__ __forceinline__ gptr<T> global_view(T *p)
+{
+   return (gptr<T>) p;
+}
+
 // Offset of canonical cell (i, j, k), 0 <= i <= nx+1 etc., in a record.
 __host__ __device__ __forceinline__ int home(const Lay &L, int i, int j, int k)
 {

@@ -103,7 +103,9 @@ struct BExItem {          // exchange: one face job, a group of variables
    int32_t m, v0;
 };
 struct BStItem {          // stencil: one block, a group of variables
-   int32_t m, slot, v0, in;
+   void *rec;             // the block's record of the first variable
+   double *part;          // its checksum partial of the first variable
+   int32_t m, nv;         // member, variables in the group
 };
 
 __global__ void k_mat_exchange(const BMember *__restrict__ mem, const int *__restrict__ pre,
@@ -124,7 +126,7 @@ __global__ void k_mat_exchange(const BMember *__restrict__ mem, const int *__res
 }
 
 __global__ void k_mat_stencil(const BMember *__restrict__ mem, const int *__restrict__ pre,
-                              int M, int vpw, BStItem *__restrict__ out)
+                              int M, int vpw, int elem_bytes, BStItem *__restrict__ out)
 {
    const int total = pre[M];
    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
@@ -132,11 +134,13 @@ __global__ void k_mat_stencil(const BMember *__restrict__ mem, const int *__rest
       const BMember &b = mem[m];
       const int local = t - pre[m], na = b.n_active;
       const int g = local / na, in = local - g * na;
+      const int v0 = g * vpw;
       BStItem it;
+      it.rec = (char *) b.pool +
+               ((int64_t) v0 * b.L.var_stride + (int64_t) b.sorted[in] * b.L.rec) * elem_bytes;
+      it.part = b.partials + (int64_t) v0 * b.part_stride + in;
       it.m = m;
-      it.slot = b.sorted[in];
-      it.v0 = g * vpw;
-      it.in = in;
+      it.nv = min(vpw, b.num_vars - v0);
       out[t] = it;
    }
 }
@@ -171,6 +175,8 @@ template <> struct Cls<3> { enum { NC = 3, NP = 4, NZ = 0, D = 4 }; };
 template <> struct Cls<4> { enum { NC = 4, NP = 6, NZ = 16, D = 4 }; };
 template <> struct Cls<5> { enum { NC = 4, NP = 6, NZ = 0, D = 4 }; };
 
+// An item's layout is its member's (a scalar load: the member index is
+// uniform across the item's threads).
 template <typename T, int ST, int VPW, int C, bool WGM>
 __global__ __launch_bounds__(256) void kb_stencil(const BMember *__restrict__ mem,
                                                   const BStItem *__restrict__ items, int total,
@@ -184,12 +190,8 @@ __global__ __launch_bounds__(256) void kb_stencil(const BMember *__restrict__ me
       for (int t = blockIdx.x; t < total; t += gridDim.x) {
          const BStItem it = items[t];
          const BMember &b = mem[it.m];
-         const int nv = min(VPW, b.num_vars - it.v0);
-         const Lay L = b.L;
-         T *rec = (T *) b.pool + (int64_t) it.v0 * L.var_stride + (int64_t) it.slot * L.rec;
-         double *part = want_partials ? b.partials + (int64_t) it.v0 * b.part_stride + it.in
-                                      : nullptr;
-         stencil_block<T, ST, 1, 2, VPW, kThreads, K::NZ>(rec, L, nv, (A *) smem, red, part,
+         stencil_block<T, ST, 1, 2, VPW, kThreads, K::NZ>((T *) it.rec, b.L, it.nv, (A *) smem,
+                                                          red, want_partials ? it.part : nullptr,
                                                           b.part_stride);
       }
    } else {
@@ -198,13 +200,9 @@ __global__ __launch_bounds__(256) void kb_stencil(const BMember *__restrict__ me
       for (int t = first_item(); t < total; t += stride) {
          const BStItem it = items[t];
          const BMember &b = mem[it.m];
-         const int nv = min(VPW, b.num_vars - it.v0);
-         const Lay L = b.L;
-         T *rec = (T *) b.pool + (int64_t) it.v0 * L.var_stride + (int64_t) it.slot * L.rec;
-         double *part = want_partials ? b.partials + (int64_t) it.v0 * b.part_stride + it.in
-                                      : nullptr;
-         stencil_block<T, ST, K::NC, K::NP, VPW, 64, K::NZ, K::D>(rec, L, nv, ring, nullptr, part,
-                                                                b.part_stride);
+         stencil_block<T, ST, K::NC, K::NP, VPW, 64, K::NZ, K::D>(
+            (T *) it.rec, b.L, it.nv, ring, nullptr, want_partials ? it.part : nullptr,
+            b.part_stride);
       }
    }
 }
@@ -450,6 +448,7 @@ int ensure_table(amr_batch *b)
       if (b->tot_st[c]) {
          hipLaunchKernelGGL(k_mat_stencil, dim3(std::min(4096, ceil_div(b->tot_st[c], 256))),
                             dim3(256), 0, b->stream, b->d_mem, b->d_pre + (3 + c) * P, M, b->vpw,
+                            b->precision == AMR_PREC_F64 ? 8 : b->precision == AMR_PREC_F32 ? 4 : 2,
                             b->d_stitems + b->st_off[c]);
          BHIP(b, hipGetLastError());
       }

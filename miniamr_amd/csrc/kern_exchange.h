@@ -32,8 +32,9 @@ __device__ __forceinline__ void exchange_job(T *__restrict__ pool, const amr_fac
    const int nb = dir == 2 ? L.ny : L.nz;   // faster in-plane axis
    const int ghost = hi ? n + 1 : 0;
    const int plane = hi ? 1 : n;            // neighbour plane next to the shared face
-   T *dst = pool + (int64_t) v0 * L.var_stride + (int64_t) jb.dst * L.rec;
-   const T *src = pool + (int64_t) v0 * L.var_stride + (int64_t) jb.src * L.rec;
+   const gptr<T> dst = global_view(pool) + (int64_t) v0 * L.var_stride + (int64_t) jb.dst * L.rec;
+   const gptr<const T> src =
+      global_view((const T *) pool) + (int64_t) v0 * L.var_stride + (int64_t) jb.src * L.rec;
    const int64_t vs = L.var_stride;
 
    if (jb.kind == AMR_FACE_SAME || jb.kind == AMR_FACE_BC) {

@@ -104,6 +104,7 @@ __device__ __forceinline__ void stencil_block(T *__restrict__ rec, const Lay &L,
    const int plane = L.gy * gz, ncol = L.ny * nz, nplanes = L.nx + 2;
    const int64_t vs = L.var_stride;
    const int tid = NT == kThreads ? (int) threadIdx.x : (int) (threadIdx.x & (NT - 1));
+   const gptr<T> __restrict__ grec = global_view(rec);
 
    // Record offsets of this thread's plane elements: plane 0, plane nx+1,
    // and interior planes p, where the offset is affine in p.
@@ -141,7 +142,7 @@ __device__ __forceinline__ void stencil_block(T *__restrict__ rec, const Lay &L,
          const int o = (p) == 0 ? off0[q] : (p) == L.nx + 1 ? offN[q]       \
                                          : off1[q] + ((p) - 1) * stp[q];    \
          _Pragma("unroll") for (int u = 0; u < VPW; u++)                   \
-            if (u < nv) pre[slot][u][q] = rec[u * vs + o];                 \
+            if (u < nv) pre[slot][u][q] = grec[u * vs + o];                \
       }                                                                    \
    }
 
@@ -181,7 +182,7 @@ __device__ __forceinline__ void stencil_block(T *__restrict__ rec, const Lay &L,
                continue;
             const A *slot = ring + (u * 3 + p % 3) * plane;
             const A *pc = ring + (u * 3 + (p + 2) % 3) * plane;   // plane p - 1
-            T *ru = rec + u * vs;
+            const gptr<T> ru = grec + u * vs;
 #pragma unroll
             for (int q = 0; q < NCOL; q++) {
                const int c = cpl[q];
@@ -233,7 +234,7 @@ __device__ __forceinline__ void stencil_block(T *__restrict__ rec, const Lay &L,
          else
             s = block_reduce(bsum[u], red);
          if (tid == 0 && u < nv)
-            partials[(int64_t) u * part_stride] = s;
+            global_view(partials)[(int64_t) u * part_stride] = s;
       }
    }
    item_sync<NT>();   // the ring (and red[]) are reused by the next item
