@@ -110,8 +110,8 @@ __global__ __launch_bounds__(256) void k_adj_stencil(T *__restrict__ lam,
    const int gy = L.gy, gz = L.gz, plane = gy * gz;
    const int bw = gz + 2, bplane = (gy + 2) * bw;     // bordered plane
    A *ring = (A *) smem;                              // 4 bordered planes
-   T *rec = lam + (int64_t) (var0 + blockIdx.y) * L.var_stride +
-            (int64_t) sorted[blockIdx.x] * L.rec;
+   const gptr<T> rec = uniform_base(global_view(lam) + (int64_t) (var0 + blockIdx.y) * L.var_stride +
+                                    (int64_t) sorted[blockIdx.x] * L.rec);
    const int tid = threadIdx.x;
    for (int e = tid; e < 4 * bplane; e += 256)
       ring[e] = A(0);
@@ -150,9 +150,9 @@ __global__ __launch_bounds__(256) void k_adj_stencil(T *__restrict__ lam,
 #pragma unroll
       for (int q = 0; q < NP; q++)
          if (tid + q * 256 < plane) {
-            pre[d][q] = rec[offset(q, p)];
+            pre[d][q] = ld(rec, offset(q, p));
             if (p >= 1 && p <= L.nx && zc1[q] >= 0)
-               zpre[d][q] = rec[zc1[q] + (p - 1) * L.ny];
+               zpre[d][q] = ld(rec, zc1[q] + (p - 1) * L.ny);
          }
    };
 #pragma unroll
@@ -226,11 +226,11 @@ __global__ __launch_bounds__(256) void k_adj_stencil(T *__restrict__ lam,
             continue;   // 7-point: edges and corners of the ghost planes
          const int o = offset(q, qo);
          if (in_plane && inner[q]) {
-            rec[o] = from_acc<T>(v);
+            st(rec, o, from_acc<T>(v));
             if (zc1[q] >= 0)
-               rec[zc1[q] + (qo - 1) * L.ny] = from_acc<T>(A(0));
+               st(rec, zc1[q] + (qo - 1) * L.ny, from_acc<T>(A(0)));
          } else {
-            rec[o] = from_acc<T>(old_prev[q] + v);
+            st(rec, o, from_acc<T>(old_prev[q] + v));
          }
       }
    }
@@ -284,6 +284,9 @@ __device__ __forceinline__ void adj_exchange_job(T *__restrict__ lam, const amr_
    const int64_t vs = L.var_stride;
    T *dst = lam + (int64_t) v0 * vs + (int64_t) jb.dst * L.rec;
    T *src = lam + (int64_t) v0 * vs + (int64_t) jb.src * L.rec;
+   // variable q's records as scalar bases (the atomic path keeps addresses)
+   auto dq = [&](int q) { return uniform_base(global_view(dst) + q * vs); };
+   auto sq = [&](int q) { return uniform_base(global_view(src) + q * vs); };
    auto srcoff = [&](int si, int sj, int sk) {
       return dir == 2 ? zcopy(L, si, sj, sk) : home(L, si, sj, sk);
    };
@@ -311,17 +314,17 @@ __device__ __forceinline__ void adj_exchange_job(T *__restrict__ lam, const amr_
 #pragma unroll
          for (int q = 0; q < KV; q++)
             if (q < nv) {
-               mu[q] = to_acc(dst[q * vs + dh]);
-               if (dc >= 0) mu[q] = mu[q] + to_acc(dst[q * vs + dc]);
-               if (!ATOMIC) sv[q] = to_acc(src[q * vs + so]);
+               mu[q] = to_acc(ld(dq(q), dh));
+               if (dc >= 0) mu[q] = mu[q] + to_acc(ld(dq(q), dc));
+               if (!ATOMIC) sv[q] = to_acc(ld(sq(q), so));
             }
 #pragma unroll
          for (int q = 0; q < KV; q++)
             if (q < nv) {
-               dst[q * vs + dh] = zero;
-               if (dc >= 0) dst[q * vs + dc] = zero;
+               st(dq(q), dh, zero);
+               if (dc >= 0) st(dq(q), dc, zero);
                if (ATOMIC) atomicAdd(&src[q * vs + so], mu[q]);
-               else src[q * vs + so] = from_acc<T>(sv[q] + mu[q]);
+               else st(sq(q), so, from_acc<T>(sv[q] + mu[q]));
             }
       }
    } else if (jb.kind == AMR_FACE_PROLONG) {
@@ -348,22 +351,22 @@ __device__ __forceinline__ void adj_exchange_job(T *__restrict__ lam, const amr_
                A m = 0;
 #pragma unroll
                for (int r = 0; r < 4; r++) {
-                  m = m + to_acc(dst[q * vs + dh[r]]);
-                  if (dc[r] >= 0) m = m + to_acc(dst[q * vs + dc[r]]);
+                  m = m + to_acc(ld(dq(q), dh[r]));
+                  if (dc[r] >= 0) m = m + to_acc(ld(dq(q), dc[r]));
                }
                mu[q] = m / A(4);
-               if (!ATOMIC) sv[q] = to_acc(src[q * vs + so]);
+               if (!ATOMIC) sv[q] = to_acc(ld(sq(q), so));
             }
 #pragma unroll
          for (int q = 0; q < KV; q++)
             if (q < nv) {
 #pragma unroll
                for (int r = 0; r < 4; r++) {
-                  dst[q * vs + dh[r]] = zero;
-                  if (dc[r] >= 0) dst[q * vs + dc[r]] = zero;
+                  st(dq(q), dh[r], zero);
+                  if (dc[r] >= 0) st(dq(q), dc[r], zero);
                }
                if (ATOMIC) atomicAdd(&src[q * vs + so], mu[q]);
-               else src[q * vs + so] = from_acc<T>(sv[q] + mu[q]);
+               else st(sq(q), so, from_acc<T>(sv[q] + mu[q]));
             }
       }
    } else {   // RESTRICT: each coarse ghost was the sum of 4 fine cells
@@ -385,23 +388,23 @@ __device__ __forceinline__ void adj_exchange_job(T *__restrict__ lam, const amr_
 #pragma unroll
          for (int q = 0; q < KV; q++)
             if (q < nv) {
-               mu[q] = to_acc(dst[q * vs + dh]);
-               if (dc >= 0) mu[q] = mu[q] + to_acc(dst[q * vs + dc]);
+               mu[q] = to_acc(ld(dq(q), dh));
+               if (dc >= 0) mu[q] = mu[q] + to_acc(ld(dq(q), dc));
                if (!ATOMIC) {
 #pragma unroll
                   for (int r = 0; r < 4; r++)
-                     sv[q][r] = to_acc(src[q * vs + so[r]]);
+                     sv[q][r] = to_acc(ld(sq(q), so[r]));
                }
             }
 #pragma unroll
          for (int q = 0; q < KV; q++)
             if (q < nv) {
-               dst[q * vs + dh] = zero;
-               if (dc >= 0) dst[q * vs + dc] = zero;
+               st(dq(q), dh, zero);
+               if (dc >= 0) st(dq(q), dc, zero);
 #pragma unroll
                for (int r = 0; r < 4; r++) {
                   if (ATOMIC) atomicAdd(&src[q * vs + so[r]], mu[q]);
-                  else src[q * vs + so[r]] = from_acc<T>(sv[q][r] + mu[q]);
+                  else st(sq(q), so[r], from_acc<T>(sv[q][r] + mu[q]));
                }
             }
       }

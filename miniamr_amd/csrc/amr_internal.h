@@ -133,6 +133,32 @@ template <typename T> __device__ __forceinline__ gptr<T> global_view(T *p)
    return (gptr<T>) p;
 }
 
+// A pointer that is the same in every lane of the wavefront, moved to
+// scalar registers.  Accesses then take the scalar base plus one 32-bit
+// vector offset (ld / st below) instead of a 64-bit address per lane and
+// per access.  Only for pointers that are wave-uniform.
+template <typename T> __device__ __forceinline__ gptr<T> uniform_base(gptr<T> p)
+{
+   const uint64_t a = (uint64_t) p;
+   const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t) a);
+   const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t) (a >> 32));
+   return (gptr<T>) (((uint64_t) hi << 32) | lo);
+}
+
+// Element i of a record-sized array (i * sizeof(T) < 2^32) at a base.
+template <typename T> __device__ __forceinline__ T ld(gptr<const T> base, uint32_t i)
+{
+   return *(gptr<const T>) ((gptr<const char>) base + i * (uint32_t) sizeof(T));
+}
+template <typename T> __device__ __forceinline__ T ld(gptr<T> base, uint32_t i)
+{
+   return *(gptr<const T>) ((gptr<const char>) base + i * (uint32_t) sizeof(T));
+}
+template <typename T> __device__ __forceinline__ void st(gptr<T> base, uint32_t i, T v)
+{
+   *(gptr<T>) ((gptr<char>) base + i * (uint32_t) sizeof(T)) = v;
+}
+
 // Offset of canonical cell (i, j, k), 0 <= i <= nx+1 etc., in a record.
 __host__ __device__ __forceinline__ int home(const Lay &L, int i, int j, int k)
 {

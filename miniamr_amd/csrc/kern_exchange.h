@@ -17,15 +17,32 @@ __device__ __forceinline__ void canon(int dir, int p, int a, int b, int &i, int 
 
 constexpr int kVars = 8;   // default variables per item, loads issued together
 
+// Thread groups a job's variables are split over: the largest power of two
+// G <= KV with G * ncell <= nthr and whole wavefronts per group, so that a
+// job with few cells (a restriction quarter, a small block's face) still
+// occupies every thread; group g takes variables g, g + G, ...  (A group's
+// variable, hence its record base, is then the same across a wavefront.)
+template <int KV>
+__device__ __forceinline__ int var_groups(int ncell, int nthr)
+{
+   int G = 1;
+   while (2 * G <= KV && 2 * G * ncell <= nthr && 2 * G * 64 <= nthr)
+      G *= 2;
+   return G;
+}
+
 // Fill the ghost plane of face job `jb` for variables v0 .. v0+nv-1
 // (nv <= KV) of the pool `pool` (the whole [var][slot][record] pool),
 // with threads tid = 0 .. nthr-1 of the item (a workgroup or a wavefront;
 // no synchronisation is needed: every thread writes its own cells).
+// A thread takes up to CP cells per pass and issues the loads of all its
+// cells and variables before the first store.
 template <typename T, int KV = kVars>
 __device__ __forceinline__ void exchange_job(T *__restrict__ pool, const amr_face_job &jb,
                                              const Lay &L, int v0, int nv, int tid, int nthr)
 {
    typedef typename Acc<T>::type A;
+   constexpr int CP = KV * sizeof(T) <= 64 ? 2 : 1;   // cells per pass (copies)
    const int dir = jb.face >> 1, hi = jb.face & 1;
    const int n = dir == 0 ? L.nx : dir == 1 ? L.ny : L.nz;
    const int na = dir == 0 ? L.ny : L.nx;   // slower in-plane axis
@@ -37,64 +54,68 @@ __device__ __forceinline__ void exchange_job(T *__restrict__ pool, const amr_fac
       global_view((const T *) pool) + (int64_t) v0 * L.var_stride + (int64_t) jb.src * L.rec;
    const int64_t vs = L.var_stride;
 
-   if (jb.kind == AMR_FACE_SAME || jb.kind == AMR_FACE_BC) {
+   if (jb.kind != AMR_FACE_RESTRICT) {
+      // SAME / BC / PROLONG: one source cell per ghost cell
       int ext_a = 0, ext_b = 0;
-      if (L.stencil == 27) {
+      if (L.stencil == 27 && jb.kind != AMR_FACE_PROLONG) {
          if (jb.kind == AMR_FACE_BC) ext_a = ext_b = 1;
          else if (dir == 1) ext_a = 1;
          else if (dir == 2) ext_a = ext_b = 1;
       }
       const int a0 = 1 - ext_a, b0 = 1 - ext_b;
-      const int wa = na + 2 * ext_a, wb = nb + 2 * ext_b;
+      const int wa = na + 2 * ext_a, wb = nb + 2 * ext_b, ncell = wa * wb;
       // BC reads the block's own plane next to the face
       const int sp = jb.kind == AMR_FACE_BC ? (hi ? n : 1) : plane;
-      for (int t = tid; t < wa * wb; t += nthr) {
-         const int a = a0 + t / wb, b = b0 + t % wb;
-         int i, j, k, si, sj, sk;
-         canon(dir, sp, a, b, si, sj, sk);
-         canon(dir, ghost, a, b, i, j, k);
-         const int so = dir == 2 ? zcopy(L, si, sj, sk) : home(L, si, sj, sk);
-         const int dh = home(L, i, j, k);
-         const int dc = (k == 1 || k == L.nz) ? zcopy(L, i, j, k) : -1;
-         T v[KV];
-#pragma unroll
-         for (int q = 0; q < KV; q++)
-            if (q < nv) v[q] = src[q * vs + so];
-#pragma unroll
-         for (int q = 0; q < KV; q++)
-            if (q < nv) {
-               dst[q * vs + dh] = v[q];
-               if (dc >= 0) dst[q * vs + dc] = v[q];
-            }
-      }
-   } else if (jb.kind == AMR_FACE_PROLONG) {
-      // fine cell (a, b) of the whole face <- coarse cell of the quarter
+      const bool prolong = jb.kind == AMR_FACE_PROLONG;
+      // prolongation: fine cell (a, b) of the whole face <- coarse cell of the quarter
       const int ao = jb.jq * (na / 2), bo = jb.iq * (nb / 2);
-      for (int t = tid; t < na * nb; t += nthr) {
-         const int a = 1 + t / nb, b = 1 + t % nb;
-         const int ca = ((a + 1) >> 1) + ao, cb = ((b + 1) >> 1) + bo;
-         int i, j, k, si, sj, sk;
-         canon(dir, plane, ca, cb, si, sj, sk);
-         canon(dir, ghost, a, b, i, j, k);
-         const int so = dir == 2 ? zcopy(L, si, sj, sk) : home(L, si, sj, sk);
-         const int dh = home(L, i, j, k);
-         const int dc = (k == 1 || k == L.nz) ? zcopy(L, i, j, k) : -1;
-         T v[KV];
+      const int G = var_groups<KV>(ncell, nthr), cw = nthr / G;
+      const int g = __builtin_amdgcn_readfirstlane(tid / cw), c0 = tid - g * cw;
+      for (int t0 = c0; t0 < ncell; t0 += CP * cw) {
+         int so[CP], dh[CP], dc[CP];
 #pragma unroll
-         for (int q = 0; q < KV; q++)
-            if (q < nv) v[q] = src[q * vs + so];
+         for (int h = 0; h < CP; h++) {
+            const int t = min(t0 + h * cw, ncell - 1);
+            const int a = a0 + t / wb, b = b0 + t % wb;
+            int i, j, k, si, sj, sk;
+            if (prolong)
+               canon(dir, sp, ((a + 1) >> 1) + ao, ((b + 1) >> 1) + bo, si, sj, sk);
+            else
+               canon(dir, sp, a, b, si, sj, sk);
+            canon(dir, ghost, a, b, i, j, k);
+            so[h] = dir == 2 ? zcopy(L, si, sj, sk) : home(L, si, sj, sk);
+            dh[h] = home(L, i, j, k);
+            dc[h] = (k == 1 || k == L.nz) ? zcopy(L, i, j, k) : -1;
+         }
+         T v[CP][KV];
 #pragma unroll
-         for (int q = 0; q < KV; q++)
-            if (q < nv) {
-               const T o = from_acc<T>(to_acc(v[q]) / A(4));
-               dst[q * vs + dh] = o;
-               if (dc >= 0) dst[q * vs + dc] = o;
-            }
+         for (int q = 0; q < KV; q++) {
+            const gptr<const T> sq = uniform_base(src + q * vs);
+#pragma unroll
+            for (int h = 0; h < CP; h++)
+               if (t0 + h * cw < ncell && q < nv && (q & (G - 1)) == g)
+                  v[h][q] = ld(sq, so[h]);
+         }
+#pragma unroll
+         for (int q = 0; q < KV; q++) {
+            const gptr<T> dq = uniform_base(dst + q * vs);
+#pragma unroll
+            for (int h = 0; h < CP; h++)
+               if (t0 + h * cw < ncell && q < nv && (q & (G - 1)) == g) {
+                  const T o = prolong ? from_acc<T>(to_acc(v[h][q]) / A(4)) : v[h][q];
+                  st(dq, dh[h], o);
+                  if (dc[h] >= 0) st(dq, dc[h], o);
+               }
+         }
       }
    } else {  // RESTRICT: coarse quarter cell <- sum of 4 fine cells
-      const int ha = na / 2, hb = nb / 2;
+      // RV of the thread's variables per pass (4 loads each)
+      constexpr int RV = CP * KV / 4 < 1 ? 1 : CP * KV / 4;
+      const int ha = na / 2, hb = nb / 2, ncell = ha * hb;
       const int ao = jb.jq * ha, bo = jb.iq * hb;
-      for (int t = tid; t < ha * hb; t += nthr) {
+      const int G = var_groups<KV>(ncell, nthr), cw = nthr / G;
+      const int g = __builtin_amdgcn_readfirstlane(tid / cw), c0 = tid - g * cw;
+      for (int t = c0; t < ncell; t += cw) {
          const int a = 1 + t / hb, b = 1 + t % hb;
          int so[4];
 #pragma unroll
@@ -107,25 +128,30 @@ __device__ __forceinline__ void exchange_job(T *__restrict__ pool, const amr_fac
          canon(dir, ghost, a + ao, b + bo, i, j, k);
          const int dh = home(L, i, j, k);
          const int dc = (k == 1 || k == L.nz) ? zcopy(L, i, j, k) : -1;
-         T v[KV][4];
+         // this thread's variables are g, g + G, ...
+         for (int q0 = g; q0 < nv; q0 += RV * G) {
+            T v[RV][4];
 #pragma unroll
-         for (int q = 0; q < KV; q++)
-            if (q < nv) {
+            for (int m = 0; m < RV; m++)
+               if (q0 + m * G < nv) {
+                  const gptr<const T> sq = uniform_base(src + (int64_t) (q0 + m * G) * vs);
 #pragma unroll
-               for (int r = 0; r < 4; r++)
-                  v[q][r] = src[q * vs + so[r]];
-            }
+                  for (int r = 0; r < 4; r++)
+                     v[m][r] = ld(sq, so[r]);
+               }
 #pragma unroll
-         for (int q = 0; q < KV; q++)
-            if (q < nv) {
-               A s = to_acc(v[q][0]);
-               s = s + to_acc(v[q][1]);
-               s = s + to_acc(v[q][2]);
-               s = s + to_acc(v[q][3]);
-               const T o = from_acc<T>(s);
-               dst[q * vs + dh] = o;
-               if (dc >= 0) dst[q * vs + dc] = o;
-            }
+            for (int m = 0; m < RV; m++)
+               if (q0 + m * G < nv) {
+                  const gptr<T> dq = uniform_base(dst + (int64_t) (q0 + m * G) * vs);
+                  A s = to_acc(v[m][0]);
+                  s = s + to_acc(v[m][1]);
+                  s = s + to_acc(v[m][2]);
+                  s = s + to_acc(v[m][3]);
+                  const T o = from_acc<T>(s);
+                  st(dq, dh, o);
+                  if (dc >= 0) st(dq, dc, o);
+               }
+         }
       }
    }
 }

@@ -104,7 +104,11 @@ __device__ __forceinline__ void stencil_block(T *__restrict__ rec, const Lay &L,
    const int plane = L.gy * gz, ncol = L.ny * nz, nplanes = L.nx + 2;
    const int64_t vs = L.var_stride;
    const int tid = NT == kThreads ? (int) threadIdx.x : (int) (threadIdx.x & (NT - 1));
-   const gptr<T> __restrict__ grec = global_view(rec);
+   // the item's records, one per variable, as scalar bases
+   gptr<T> ub[VPW];
+#pragma unroll
+   for (int u = 0; u < VPW; u++)
+      ub[u] = uniform_base(global_view(rec) + u * vs);
 
    // Record offsets of this thread's plane elements: plane 0, plane nx+1,
    // and interior planes p, where the offset is affine in p.
@@ -142,7 +146,7 @@ __device__ __forceinline__ void stencil_block(T *__restrict__ rec, const Lay &L,
          const int o = (p) == 0 ? off0[q] : (p) == L.nx + 1 ? offN[q]       \
                                          : off1[q] + ((p) - 1) * stp[q];    \
          _Pragma("unroll") for (int u = 0; u < VPW; u++)                   \
-            if (u < nv) pre[slot][u][q] = grec[u * vs + o];                \
+            if (u < nv) pre[slot][u][q] = ld(ub[u], o);                    \
       }                                                                    \
    }
 
@@ -182,7 +186,6 @@ __device__ __forceinline__ void stencil_block(T *__restrict__ rec, const Lay &L,
                continue;
             const A *slot = ring + (u * 3 + p % 3) * plane;
             const A *pc = ring + (u * 3 + (p + 2) % 3) * plane;   // plane p - 1
-            const gptr<T> ru = grec + u * vs;
 #pragma unroll
             for (int q = 0; q < NCOL; q++) {
                const int c = cpl[q];
@@ -214,9 +217,9 @@ __device__ __forceinline__ void stencil_block(T *__restrict__ rec, const Lay &L,
                }
                if (out_now) {
                   const T out = from_acc<T>(w);
-                  ru[(i - 1) * ncol + tid + q * NT] = out;
+                  st(ub[u], (i - 1) * ncol + tid + q * NT, out);
                   if (zc1[q] >= 0)
-                     ru[zc1[q] + (i - 1) * L.ny] = out;
+                     st(ub[u], zc1[q] + (i - 1) * L.ny, out);
                   if (partials)
                      bsum[u] += (double) to_acc(out);
                }
