@@ -146,7 +146,7 @@ __device__ __forceinline__ void stencil_block(T *__restrict__ rec, const Lay &L,
          const int o = (p) == 0 ? off0[q] : (p) == L.nx + 1 ? offN[q]       \
                                          : off1[q] + ((p) - 1) * stp[q];    \
          _Pragma("unroll") for (int u = 0; u < VPW; u++)                   \
-            if (u < nv) pre[slot][u][q] = ld(ub[u], o);                    \
+            if (u < nv) pre[slot][u][q] = ld_stream(ub[u], o);             \
       }                                                                    \
    }
 
@@ -217,9 +217,9 @@ __device__ __forceinline__ void stencil_block(T *__restrict__ rec, const Lay &L,
                }
                if (out_now) {
                   const T out = from_acc<T>(w);
-                  st(ub[u], (i - 1) * ncol + tid + q * NT, out);
+                  st_stream(ub[u], (i - 1) * ncol + tid + q * NT, out);
                   if (zc1[q] >= 0)
-                     st(ub[u], zc1[q] + (i - 1) * L.ny, out);
+                     st_stream(ub[u], zc1[q] + (i - 1) * L.ny, out);
                   if (partials)
                      bsum[u] += (double) to_acc(out);
                }
