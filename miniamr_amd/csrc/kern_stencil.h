@@ -110,13 +110,32 @@ __device__ __forceinline__ void stencil_block(T *__restrict__ rec, const Lay &L,
    for (int u = 0; u < VPW; u++)
       ub[u] = uniform_base(global_view(rec) + u * vs);
 
-   // Record offsets of this thread's plane elements: plane 0, plane nx+1,
-   // and interior planes p, where the offset is affine in p.
-   int off0[NPER], offN[NPER], off1[NPER], stp[NPER];
+   // This thread's plane elements r = tid + q*NT: the ny*nz interior cells
+   // first, in record order (a wavefront's loads then cover whole cache
+   // lines), then the ghost ring: rows j = 0 and j = ny+1, columns k = 0
+   // and k = nz+1.  pos is the element's place in an LDS plane; the record
+   // offsets are those of plane 0, plane nx+1 and interior planes p, where
+   // the offset is affine in p.
+   int off0[NPER], offN[NPER], off1[NPER], stp[NPER], pos[NPER];
 #pragma unroll
    for (int q = 0; q < NPER; q++) {
-      const int e = min(tid + q * NT, plane - 1);
-      const int jj = e / gz, kk = e - jj * gz;
+      const int r = min(tid + q * NT, plane - 1);
+      int jj, kk;
+      if (r < ncol) {
+         jj = r / nz + 1;
+         kk = r - (jj - 1) * nz + 1;
+      } else {
+         const int g = r - ncol;
+         if (g < 2 * gz) {
+            jj = g < gz ? 0 : L.ny + 1;
+            kk = g < gz ? g : g - gz;
+         } else {
+            const int h = g - 2 * gz;
+            jj = (h < L.ny ? h : h - L.ny) + 1;
+            kk = h < L.ny ? 0 : nz + 1;
+         }
+      }
+      pos[q] = jj * gz + kk;
       off0[q] = home(L, 0, jj, kk);
       offN[q] = home(L, L.nx + 1, jj, kk);
       off1[q] = home(L, 1, jj, kk);
@@ -171,9 +190,8 @@ __device__ __forceinline__ void stencil_block(T *__restrict__ rec, const Lay &L,
             A *slot = ring + (u * 3 + p % 3) * plane;
 #pragma unroll
             for (int q = 0; q < NPER; q++) {
-               const int e = tid + q * NT;
-               if (e < plane && u < nv)
-                  slot[e] = to_acc(pre[d][u][q]);
+               if (tid + q * NT < plane && u < nv)
+                  slot[pos[q]] = to_acc(pre[d][u][q]);
             }
          }
          if (p + D < nplanes) { PREFETCH(d, p + D); }
