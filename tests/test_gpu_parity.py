@@ -43,13 +43,19 @@ def assert_same_cells(g, c, slots):
             s, u.max(), np.unravel_index(int(u.argmax()), u.shape))
 
 
+# Block shapes: small faces (variables split over wavefront groups), the
+# 16^3 workload's, and uneven ones; 11 variables = one full and one partial
+# group of 8.
 @pytest.mark.parametrize("st", [7, 27])
 @pytest.mark.parametrize("permute", [False, True])
-def test_exchange_matches_oracle(libs, st, permute):
-    print("case: exchange stencil=%d permute=%s" % (st, permute), flush=True)
+@pytest.mark.parametrize("n,nv,vars_", [((6, 4, 8), 3, 2), ((16, 16, 16), 11, 11),
+                                        ((12, 10, 14), 11, 11)])
+def test_exchange_matches_oracle(libs, st, permute, n, nv, vars_):
+    print("case: exchange stencil=%d permute=%s block=%s vars=%d" % (st, permute, n, vars_),
+          flush=True)
     rng = np.random.default_rng(10 + st + permute)
     n_src, n_dst = 5, 6
-    kw = dict(nx=6, ny=4, nz=8, num_vars=3, max_blocks=n_src + n_dst + 2, stencil=st)
+    kw = dict(nx=n[0], ny=n[1], nz=n[2], num_vars=nv, max_blocks=n_src + n_dst + 2, stencil=st)
     g, c = pair(libs, **kw)
     for s in range(n_src + n_dst):
         cells = rng.random(g.shape)
@@ -61,12 +67,13 @@ def test_exchange_matches_oracle(libs, st, permute):
         d.set_mesh(order, plan)
     for stage in range(3):
         for d in (g, c):
-            d.comm(0, 2, stage=stage, permute=permute)   # variables 0-1 only
+            d.comm(0, vars_, stage=stage, permute=permute)
     assert_same_cells(g, c, range(n_src + n_dst))
 
 
 @pytest.mark.parametrize("st", [7, 27])
-@pytest.mark.parametrize("n", [(4, 4, 4), (10, 10, 10), (16, 16, 16), (6, 2, 8)])
+@pytest.mark.parametrize("n", [(4, 4, 4), (10, 10, 10), (16, 16, 16), (6, 2, 8), (20, 20, 20),
+                               (6, 30, 30)])
 def test_stencil_and_checksum_match_oracle(libs, st, n):
     print("case: stencil=%d block=%s" % (st, n), flush=True)
     rng = np.random.default_rng(20 + st)
