@@ -97,17 +97,23 @@ int aerr(amr_adj *a, int rc, const char *fmt, ...)
 // (the forward overwrote them) and their copies clear; ghost cells add it
 // (7-point: face ghosts only, nothing reads edges and corners).
 //
-// Each thread owns plane positions e = tid + q * 256 (q < NP); t planes sit
-// in a 4-deep LDS ring with a zero border, so neighbour reads need no
-// bounds tests; the ghost cells' old adjoints are kept from their load.
-template <typename T, int ST, int NP>
+// Each thread owns plane positions r = tid + q * 256 (q < NP), the ny*nz
+// interior positions first in record order, then the ghost ring (as the
+// forward kernel, so a wavefront's loads cover whole cache lines); only the
+// first IQ slots can hold interior positions.  t planes sit in a 4-deep LDS
+// ring with a zero border, so neighbour reads need no bounds tests; the
+// ghost cells' old adjoints are kept from their load.  NZ, when not 0, is
+// the block's nz at compile time.  t = mu / DIV uses the same division as
+// the forward kernel (IEEE in FP64, a corrected reciprocal product in FP32).
+template <typename T, int ST, int NP, int IQ, int NZ>
 __global__ __launch_bounds__(256) void k_adj_stencil(T *__restrict__ lam,
                                                       const int32_t *__restrict__ sorted, Lay L,
                                                       int var0)
 {
    typedef typename Acc<T>::type A;
    extern __shared__ __align__(16) unsigned char smem[];
-   const int gy = L.gy, gz = L.gz, plane = gy * gz;
+   const int nz = NZ ? NZ : L.nz, gz = nz + 2, gy = L.gy, plane = gy * gz;
+   const int ncol = L.ny * nz;
    const int bw = gz + 2, bplane = (gy + 2) * bw;     // bordered plane
    A *ring = (A *) smem;                              // 4 bordered planes
    const gptr<T> rec = uniform_base(global_view(lam) + (int64_t) (var0 + blockIdx.y) * L.var_stride +
@@ -116,28 +122,43 @@ __global__ __launch_bounds__(256) void k_adj_stencil(T *__restrict__ lam,
    for (int e = tid; e < 4 * bplane; e += 256)
       ring[e] = A(0);
 
-   int off0[NP], offN[NP], off1[NP], stp[NP], zc1[NP], bpos[NP];
-   bool inner[NP], live[NP];
+   int off0[NP], offN[NP], off1[NP], stp[NP], zc1[IQ], bpos[NP];
+   bool live[NP];
 #pragma unroll
    for (int q = 0; q < NP; q++) {
-      const int e = min(tid + q * 256, plane - 1);
-      const int j = e / gz, k = e - j * gz;
+      const int r = min(tid + q * 256, plane - 1);
+      int j, k;
+      if (r < ncol) {
+         j = r / nz + 1;
+         k = r - (j - 1) * nz + 1;
+      } else {
+         const int g = r - ncol;
+         if (g < 2 * gz) {
+            j = g < gz ? 0 : L.ny + 1;
+            k = g < gz ? g : g - gz;
+         } else {
+            const int h = g - 2 * gz;
+            j = (h < L.ny ? h : h - L.ny) + 1;
+            k = h < L.ny ? 0 : nz + 1;
+         }
+      }
       live[q] = tid + q * 256 < plane;
       off0[q] = home(L, 0, j, k);
       offN[q] = home(L, L.nx + 1, j, k);
       off1[q] = home(L, 1, j, k);
       stp[q] = L.nx > 1 ? home(L, 2, j, k) - off1[q] : 0;
-      inner[q] = j >= 1 && j <= L.ny && k >= 1 && k <= L.nz;
-      zc1[q] = inner[q] && (k == 1 || k == L.nz) ? zcopy(L, 1, j, k) : -1;
+      if (q < IQ)
+         zc1[q] = r < ncol && (k == 1 || k == nz) ? zcopy(L, 1, j, k) : -1;
       bpos[q] = (j + 1) * bw + (k + 1);
-      if (ST == 7 && !inner[q] && (j == 0 || j == gy - 1) && (k == 0 || k == gz - 1))
+      if (ST == 7 && r >= ncol && (j == 0 || j == gy - 1) && (k == 0 || k == gz - 1))
          live[q] = false;   // 7-point: edge lines of interior planes are not touched
    }
+   // interior positions: the first ncol of the enumeration
+   auto inner = [&](int q) { return q < IQ && tid + q * 256 < ncol; };
    auto offset = [&](int q, int p) {
       return p == 0 ? off0[q] : p == L.nx + 1 ? offN[q] : off1[q] + (p - 1) * stp[q];
    };
    const int np = L.nx + 2;
-   const A div = ST == 27 ? A(27) : A(7);
    A s_prev[NP], s_cur[NP], old_prev[NP], old_cur[NP];
 #pragma unroll
    for (int q = 0; q < NP; q++)
@@ -145,14 +166,14 @@ __global__ __launch_bounds__(256) void k_adj_stencil(T *__restrict__ lam,
    // planes of loads in flight: the adjoint of every cell of the plane and,
    // for interior rows at z = 1 / z = nz, of its copy
    constexpr int D = kDepth;
-   T pre[D][NP], zpre[D][NP];
+   T pre[D][NP], zpre[D][IQ];
    auto fetch = [&](int d, int p) {
 #pragma unroll
       for (int q = 0; q < NP; q++)
          if (tid + q * 256 < plane) {
             pre[d][q] = ld(rec, offset(q, p));
-            if (p >= 1 && p <= L.nx && zc1[q] >= 0)
-               zpre[d][q] = ld(rec, zc1[q] + (p - 1) * L.ny);
+            if (q < IQ && p >= 1 && p <= L.nx && zc1[q < IQ ? q : 0] >= 0)
+               zpre[d][q < IQ ? q : 0] = ld(rec, zc1[q < IQ ? q : 0] + (p - 1) * L.ny);
          }
    };
 #pragma unroll
@@ -173,11 +194,11 @@ __global__ __launch_bounds__(256) void k_adj_stencil(T *__restrict__ lam,
          A old = A(0), t = A(0);
          if (p < np && tid + q * 256 < plane) {
             old = to_acc(pre[d][q]);
-            if (p >= 1 && p <= L.nx && inner[q]) {
+            if (p >= 1 && p <= L.nx && inner(q)) {
                A mu = old;
-               if (zc1[q] >= 0)
-                  mu = mu + to_acc(zpre[d][q]);
-               t = mu / div;
+               if (zc1[q < IQ ? q : 0] >= 0)
+                  mu = mu + to_acc(zpre[d][q < IQ ? q : 0]);
+               t = divide<A, ST>(mu);
             }
          }
          if (tid + q * 256 < plane)
@@ -222,13 +243,13 @@ __global__ __launch_bounds__(256) void k_adj_stencil(T *__restrict__ lam,
          if (!live[q])
             continue;
          const bool in_plane = qo >= 1 && qo <= L.nx;
-         if (ST == 7 && !in_plane && !inner[q])
+         if (ST == 7 && !in_plane && !inner(q))
             continue;   // 7-point: edges and corners of the ghost planes
          const int o = offset(q, qo);
-         if (in_plane && inner[q]) {
+         if (in_plane && inner(q)) {
             st(rec, o, from_acc<T>(v));
-            if (zc1[q] >= 0)
-               st(rec, zc1[q] + (qo - 1) * L.ny, from_acc<T>(A(0)));
+            if (zc1[q < IQ ? q : 0] >= 0)
+               st(rec, zc1[q < IQ ? q : 0] + (qo - 1) * L.ny, from_acc<T>(A(0)));
          } else {
             st(rec, o, from_acc<T>(old_prev[q] + v));
          }
@@ -616,14 +637,21 @@ int adj_stencil(amr_adj *a, int start, int num)
       typedef typename Acc<T>::type A;
       const size_t lds = sizeof(A) * 4 * (size_t) (d->g.gy + 2) * (d->g.gz + 2);
       dim3 grid(a->n_active, num);
-#define GO(ST, NP) \
-   hipLaunchKernelGGL((k_adj_stencil<T, ST, NP>), grid, dim3(256), lds, d->stream, \
+      const int ncol = d->g.ny * d->g.nz;
+      // NP plane positions per thread, IQ of them possibly interior; a
+      // compile-time nz for the 16^3 and 8^3 shapes
+#define GO(ST, NP, IQ, NZ) \
+   hipLaunchKernelGGL((k_adj_stencil<T, ST, NP, IQ, NZ>), grid, dim3(256), lds, d->stream, \
                       (T *) a->pool, a->d_sorted, d->g, start)
-      if (d->g.stencil == 27) {
-         if (plane <= 256) GO(27, 1); else if (plane <= 512) GO(27, 2); else GO(27, 3);
-      } else {
-         if (plane <= 256) GO(7, 1); else if (plane <= 512) GO(7, 2); else GO(7, 3);
-      }
+#define BY_SHAPE(ST)                                                                   \
+   if (d->g.nz == 16 && d->g.ny == 16) GO(ST, 2, 1, 16);                              \
+   else if (d->g.nz == 8 && d->g.ny == 8) GO(ST, 1, 1, 8);                            \
+   else if (plane <= 256) GO(ST, 1, 1, 0);                                            \
+   else if (plane <= 512) { if (ncol <= 256) GO(ST, 2, 1, 0); else GO(ST, 2, 2, 0); } \
+   else { if (ncol <= 256) GO(ST, 3, 1, 0); else if (ncol <= 512) GO(ST, 3, 2, 0);     \
+          else GO(ST, 3, 3, 0); }
+      if (d->g.stencil == 27) { BY_SHAPE(27) } else { BY_SHAPE(7) }
+#undef BY_SHAPE
 #undef GO
       AHIP(a, hipGetLastError());
       return AMR_OK;
