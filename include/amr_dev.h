@@ -38,6 +38,7 @@
 #ifndef AMR_DEV_H
 #define AMR_DEV_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -158,6 +159,16 @@ int amr_dev_sync(amr_dev *dev);
  * [3] split/consolidate.  Requires amr_dev_set_timing(dev, 1). */
 int amr_dev_set_timing(amr_dev *dev, int on);
 int amr_dev_read_timing(amr_dev *dev, double ms[4]);
+/* The exchange time of ms[0] split by direction (x, y, z), accumulated by
+ * amr_dev_read_timing since the last call of this one.  For the end-of-run
+ * report's per-direction rows (ref/profile.c:1085-1166); no counterpart
+ * in the reference's call interface. */
+int amr_dev_read_exchange_timing(amr_dev *dev, double ms[3]);
+
+/* Bytes of device (and pinned host) memory the context holds, and in how
+ * many buffers: the report's memory lines (ref/profile.c:1042-1053), whose
+ * reference counterpart counts the program's own mallocs. */
+int amr_dev_memory(const amr_dev *dev, size_t *bytes, int *buffers);
 
 #ifdef __cplusplus
 }

@@ -34,7 +34,7 @@ ABI_SYMBOLS = [
     "amr_dev_download_block", "amr_dev_set_mesh", "amr_dev_comm", "amr_dev_stencil",
     "amr_dev_check_sum", "amr_dev_split", "amr_dev_consolidate", "amr_dev_reserve_sums",
     "amr_dev_read_sums", "amr_dev_stage", "amr_dev_sync", "amr_dev_set_timing",
-    "amr_dev_read_timing",
+    "amr_dev_read_timing", "amr_dev_read_exchange_timing", "amr_dev_memory",
 ]
 
 
@@ -103,6 +103,8 @@ def load_library(path=LIB_PATH):
         "amr_dev_sync": (ip, [vp]),
         "amr_dev_set_timing": (ip, [vp, ip]),
         "amr_dev_read_timing": (ip, [vp, dp]),
+        "amr_dev_read_exchange_timing": (ip, [vp, dp]),
+        "amr_dev_memory": (ip, [vp, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ip)]),
         # include/amr_batch.h, device half
         "amr_batch_create": (ip, [ip, ip, ip, ctypes.POINTER(vp)]),
         "amr_batch_destroy": (ip, [vp]),

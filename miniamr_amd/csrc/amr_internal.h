@@ -90,6 +90,7 @@ struct amr_dev {
    bool replaying = false;          // calls come from the tape: do not record
 
    bool timing = false;
+   double exchange_ms[3] = {0.0, 0.0, 0.0};   // per direction, read by read_timing
    std::vector<hipEvent_t> ev_free;
    struct Span { hipEvent_t a, b; int cat; };
    std::vector<Span> spans;
@@ -234,6 +235,9 @@ int amr_hip_check(amr_dev *d, hipError_t e, const char *what);
 // 3 split/merge.
 void amr_span_begin(amr_dev *d, int cat);
 void amr_span_end(amr_dev *d);
+// Span categories: 0 exchange (the batch's graph), 1 stencil, 2 checksum,
+// 3 split/consolidate, kSpanExchangeX + dir one exchange direction.
+constexpr int kSpanExchangeX = 4;
 
 // Unrecorded implementations behind the public calls (context.hip), used
 // by the tape's replay (adjoint.hip).

@@ -249,14 +249,14 @@ int amr_comm_impl(amr_dev *d, int start, int num, int stage, int permute)
                            start + num);
    if (!num)
       return AMR_OK;
-   amr_span_begin(d, 0);
    for (int o = 0; o < 3; o++) {
       int dir = permute ? perm[stage % 6][o] : o;
+      amr_span_begin(d, kSpanExchangeX + dir);
       int rc = amr_launch_exchange_dir(d, dir, start, num);
+      amr_span_end(d);
       if (rc)
          return rc;
    }
-   amr_span_end(d);
    return AMR_OK;
 }
 
@@ -523,12 +523,53 @@ int amr_dev_read_timing(amr_dev *d, double ms[4])
    HIP_TRY(d, hipStreamSynchronize(d->stream));
    for (auto &s : d->spans) {
       float t = 0.0f;
-      if (s.a && s.b && hipEventElapsedTime(&t, s.a, s.b) == hipSuccess)
-         ms[s.cat] += t;
+      if (s.a && s.b && hipEventElapsedTime(&t, s.a, s.b) == hipSuccess) {
+         if (s.cat >= kSpanExchangeX) {
+            ms[0] += t;                                   // exchange, by direction
+            d->exchange_ms[s.cat - kSpanExchangeX] += t;
+         } else {
+            ms[s.cat] += t;
+         }
+      }
       if (s.a) d->ev_free.push_back(s.a);
       if (s.b) d->ev_free.push_back(s.b);
    }
    d->spans.clear();
+   return AMR_OK;
+}
+
+int amr_dev_read_exchange_timing(amr_dev *d, double ms[3])
+{
+   if (!d || !ms)
+      return AMR_ERR_ARG;
+   for (int i = 0; i < 3; i++) {
+      ms[i] = d->exchange_ms[i];
+      d->exchange_ms[i] = 0.0;
+   }
+   return AMR_OK;
+}
+
+int amr_dev_memory(const amr_dev *d, size_t *bytes, int *buffers)
+{
+   if (!d || !bytes)
+      return AMR_ERR_ARG;
+   const size_t b[] = {
+      d->pool ? (size_t) d->g.var_stride * d->c.num_vars * d->elem_bytes : 0,
+      d->d_sorted ? sizeof(int32_t) * d->c.max_blocks : 0,
+      sizeof(amr_face_job) * (size_t) d->jobs_cap,
+      d->d_partials ? sizeof(double) * (size_t) d->c.num_vars * d->c.max_blocks : 0,
+      sizeof(double) * (size_t) d->sum_rows * d->c.num_vars,
+      d->d_stage ? sizeof(double) * (size_t) d->g.gx * d->g.gy * d->g.gz * d->c.num_vars : 0,
+      d->scratch_bytes, d->jobbuf_bytes, d->h_pinned ? d->h_pinned_bytes : 0 };
+   size_t tot = 0;
+   int n = 0;
+   for (size_t x : b) {
+      tot += x;
+      n += x > 0;
+   }
+   *bytes = tot;
+   if (buffers)
+      *buffers = n;
    return AMR_OK;
 }
 

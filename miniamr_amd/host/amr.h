@@ -116,6 +116,18 @@ typedef struct amr_run {
 
    /* timers (seconds) in the reference's report slots */
    double t_all, t_init, t_refine_all, t_comm, t_calc, t_cs, t_main;
+   double t_comm_dir[3];       /* exchange device time per direction */
+   double t_cs_dev, t_cs_host; /* checksum: reduction kernels, host check */
+   double t_refine_init;       /* the initial refinement */
+   double t_ref_compare, t_ref_mark, t_ref_split, t_ref_coarsen;
+   int nrs;                    /* refinement steps (ref/refine.c:45) */
+   /* faces exchanged per direction, counted per comm call as the
+    * reference does (ref/comm.c:169-196), and the current plan's counts */
+   int64_t face_same[3], face_diff[3], face_bc[3];
+   int32_t plan_faces[3][3];   /* [dir][same, coarse/fine, boundary] */
+   /* device memory (amr_dev_memory): after initialisation, at the end */
+   size_t mem_bytes, mem_bytes_init;
+   int mem_buffers, mem_buffers_init;
 
    /* The reference draws its initial values from libc rand() with the
     * implicit seed 1 (ref/init.c:494).  The GPU runtime calls srand()/rand()
@@ -180,7 +192,8 @@ void amr_driver_end(amr_run *R);
 void amr_driver_set_sum_log(amr_run *R, const char *path);
 int amr_dump_final_to(amr_run *R, const char *cells_path, const char *bsums_path);
 double amr_calc_time_step(amr_run *R);
-void amr_report(amr_run *R);
+void amr_report(amr_run *R);                 /* report.c */
+void amr_count_comm(amr_run *R, int calls);  /* driver.c */
 int amr_dump_final(amr_run *R);
 double amr_checksum(amr_run *R, int var);
 

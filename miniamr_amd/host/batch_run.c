@@ -136,6 +136,8 @@ int amr_batch_run_step(amr_batch_run *B)
          amr_run *R = B->runs[i];
          amr_driver_step_advance(R);
          R->loop.stage_calls += R->P.stages_per_ts;
+         amr_count_comm(R, R->P.stages_per_ts *
+                              ((R->P.num_vars + R->P.comm_vars - 1) / R->P.comm_vars));
          if (amr_driver_step_end(R) == 2)
             B->stopped = 1;
       }

@@ -100,6 +100,7 @@ static void enqueue_timestep(amr_run *R, int comm_stage0)
             amr_dev_check(R, amr_dev_stage(R->dev, start, number, comm_stage0 + stage,
                                            P->permute, row), "stage");
             R->loop.stage_calls++;
+            amr_count_comm(R, 1);
          }
       }
    } else {
@@ -125,8 +126,19 @@ static void enqueue_timestep(amr_run *R, int comm_stage0)
             amr_dev_check(R, amr_dev_stage(R->dev, start, number, comm_stage0 + stage,
                                            P->permute, row), "stage");
             R->loop.stage_calls++;
+            amr_count_comm(R, 1);
          }
       }
+   }
+}
+
+/* Faces exchanged by `calls` comm calls on the current plan. */
+void amr_count_comm(amr_run *R, int calls)
+{
+   for (int d = 0; d < 3; d++) {
+      R->face_same[d] += (int64_t) calls * R->plan_faces[d][0];
+      R->face_diff[d] += (int64_t) calls * R->plan_faces[d][1];
+      R->face_bc[d] += (int64_t) calls * R->plan_faces[d][2];
    }
 }
 
@@ -176,12 +188,14 @@ int amr_driver_begin(amr_run *R)
       log_sum(R, R->grid_sum[v]);
    double t1 = amr_now();
    R->t_init = t1 - t2;
+   R->total_red = 0;   /* the initial sums are not counted (ref/driver.c:45-46) */
+   amr_dev_memory(R->dev, &R->mem_bytes_init, &R->mem_buffers_init);
 
    if (P->num_refine || P->uniform_refine)
       amr_refine(R, 0);
    amr_dev_check(R, amr_dev_sync(R->dev), "sync");
    t2 = amr_now();
-   R->t_refine_all += t2 - t1;
+   R->t_refine_all += R->t_refine_init = t2 - t1;
    R->nb_min = R->nb_max = R->global_active;
 
    L->n_rows = P->checksum_freq ? (P->stages_per_ts + P->checksum_freq - 1) / P->checksum_freq : 0;
@@ -249,7 +263,7 @@ int amr_driver_step_end(amr_run *R)
       return 2;
    }
    double t4 = amr_now();
-   R->t_cs += t4 - t3;
+   R->t_cs_host += t4 - t3;
 
    if (P->num_refine && !P->uniform_refine) {
       amr_move_objects(R, L->delta);
@@ -302,7 +316,13 @@ void amr_driver_end(amr_run *R)
       L->dev_ms[i] += ms[i];
    R->t_comm = L->dev_ms[0] * 1e-3;
    R->t_calc = L->dev_ms[1] * 1e-3;
-   R->t_cs += L->dev_ms[2] * 1e-3;
+   R->t_cs_dev = L->dev_ms[2] * 1e-3;
+   R->t_cs = R->t_cs_dev + R->t_cs_host;
+   double dir_ms[3];
+   amr_dev_read_exchange_timing(R->dev, dir_ms);
+   for (int d = 0; d < 3; d++)
+      R->t_comm_dir[d] = dir_ms[d] * 1e-3;
+   amr_dev_memory(R->dev, &R->mem_bytes, &R->mem_buffers);
    if (L->state != 2) {
       P->end_time = L->sim_time;
       P->num_tsteps = L->ts - 1;
@@ -322,40 +342,6 @@ int amr_driver(amr_run *R)
       ;
    amr_driver_end(R);
    return rc == 2;
-}
-
-/* Summary line of ref/profile.c:1298 plus this build's throughput line. */
-void amr_report(amr_run *R)
-{
-   const amr_params *P = &R->P;
-   double stages = (double) P->num_tsteps * P->stages_per_ts;
-   double cells = (double) P->nx * P->ny * P->nz;
-   double updates = (double) R->total_blocks * cells * P->num_vars;
-   static const char *prec_name[3] = { "f64", "f32", "bf16" };
-   if (P->report_perf & 4) {
-      printf("\n ================ Start report ===================\n\n");
-      printf("          miniAMR hot path on MI355X (gfx950), one rank, one GPU\n\n");
-      printf("block size %d x %d x %d, initial blocks %d x %d x %d, %d variables\n",
-             P->nx, P->ny, P->nz, P->init_block_x, P->init_block_y, P->init_block_z,
-             P->num_vars);
-      printf("stencil %d, storage %s, schedule %s\n", P->stencil,
-             prec_name[P->precision], P->schedule ? "variable-major" : "stage-major");
-      printf("timesteps %d, stages per timestep %d, blocks per stage %.2lf (min %lld max %lld)\n",
-             P->num_tsteps, P->stages_per_ts,
-             stages > 0 ? (double) R->total_blocks / stages : 0.0,
-             (long long) R->nb_min, (long long) R->nb_max);
-      printf("blocks split %lld, blocks consolidated %lld\n",
-             (long long) R->num_refined, (long long) R->num_reformed);
-      printf("time: total %lf init %lf refine %lf; device: exchange %lf stencil %lf checksum %lf\n",
-             R->t_all, R->t_init, R->t_refine_all, R->t_comm, R->t_calc, R->t_cs);
-      if (R->t_all > 0.0)
-         printf("cell-var updates %.6e, %.4lf GCell-var updates/s over the timestep loop\n",
-                updates, updates / R->t_all * 1e-9);
-      printf("\n ================== End report ===================\n");
-   }
-   printf("Summary: ranks %d ts %d time %lf calc %lf comm %lf red %lf refine %lf blocks/ts %lf max_blocks %d\n",
-          1, P->num_tsteps, R->t_all, R->t_calc, R->t_comm, R->t_cs, R->t_refine_all,
-          stages > 0 ? ((double) R->total_blocks) / stages : 0.0, R->local_max_b);
 }
 
 /* Final-state dumps in the formats of the golden-vector harness. */

@@ -335,6 +335,16 @@ int amr_build_exchange_plan(amr_run *R)
       }
       count[dir] = v->n - before;
    }
+   /* faces per comm call for the report: same level 2 per pair (one job per
+    * side), coarse/fine 2 per quarter pair (restriction + prolongation job),
+    * 1 per boundary face */
+   memset(R->plan_faces, 0, sizeof(R->plan_faces));
+   for (int t = 0, dir = 0; dir < 3; dir++)
+      for (int e = 0; e < count[dir]; e++, t++) {
+         const amr_face_job *jj = (const amr_face_job *) v->v + t;
+         int k = jj->kind == AMR_FACE_SAME ? 0 : jj->kind == AMR_FACE_BC ? 2 : 1;
+         R->plan_faces[dir][k]++;
+      }
    return amr_dev_check(R, amr_dev_set_mesh(R->dev, R->sorted_slot, na,
                                             (const amr_face_job *) v->v, count),
                         "set_mesh");

@@ -372,6 +372,7 @@ void amr_refine(amr_run *R, int ts)
 {
    const amr_params *P = &R->P;
    int steps = ts ? P->block_change : P->num_refine;
+   R->nrs++;
 
    for (int step = 0; step < steps; step++) {
       for (int j = P->num_refine; j >= 0; j--)
@@ -379,6 +380,7 @@ void amr_refine(amr_run *R, int ts)
             R->cur_max_level = j;
             break;
          }
+      double t0 = amr_now();
       reset_all(R);
       if (P->uniform_refine) {
          int na = amr_n_active(R);
@@ -389,9 +391,17 @@ void amr_refine(amr_run *R, int ts)
       } else {
          amr_check_objects(R);
       }
+      double t1 = amr_now();
       amr_refine_level(R);
+      double t2 = amr_now();
       amr_split_blocks(R);
+      double t3 = amr_now();
       amr_consolidate_blocks(R);
+      double t4 = amr_now();
+      R->t_ref_compare += t1 - t0;
+      R->t_ref_mark += t2 - t1;
+      R->t_ref_split += t3 - t2;
+      R->t_ref_coarsen += t4 - t3;
    }
    if (R->num_active > R->local_max_b)
       R->local_max_b = R->num_active;

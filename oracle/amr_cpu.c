@@ -359,6 +359,23 @@ int amr_dev_read_timing(amr_dev *d, double ms[4])
    return AMR_OK;
 }
 
+int amr_dev_read_exchange_timing(amr_dev *d, double ms[3])
+{
+   (void) d;
+   ms[0] = ms[1] = ms[2] = 0.0;
+   return AMR_OK;
+}
+
+int amr_dev_memory(const amr_dev *d, size_t *bytes, int *buffers)
+{
+   if (!d || !bytes)
+      return AMR_ERR_ARG;
+   *bytes = 0;
+   if (buffers)
+      *buffers = 0;
+   return AMR_OK;
+}
+
 /* ------------------------------------------------------------------------
  * include/amr_batch.h on the CPU: the batch's definition restated as "each
  * member steps alone" (members in order, all variables per stage), which

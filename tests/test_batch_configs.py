@@ -57,7 +57,7 @@ def topology(args):
     for flag, val in (("--num_vars", "1"), ("--stages_per_ts", "1"), ("--checksum_freq", "1"),
                       ("--precision", "f64")):
         a[a.index(flag) + 1] = val
-    out = subprocess.run([A.ORACLE_X] + a + ["--report_perf", "8"], stdout=subprocess.PIPE,
+    out = subprocess.run([A.ORACLE_X] + a + ["--report_perf", "12"], stdout=subprocess.PIPE,
                          timeout=300).stdout.decode()
     tot = [int(x) for x in re.findall(r"Total number of blocks at timestep \d+ is (\d+)", out)]
     m = re.search(r"max_blocks (\d+)", out)

@@ -21,7 +21,8 @@ import miniamr_amd as M
 
 pytestmark = pytest.mark.gpu
 
-CASES = sorted(f[:-5] for f in os.listdir(A.GOLDEN) if f.endswith(".json"))
+CASES = sorted(f[:-5] for f in os.listdir(A.GOLDEN)
+               if f.endswith(".json") and not f.startswith("report_"))
 
 
 @pytest.fixture(scope="module")

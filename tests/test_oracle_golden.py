@@ -16,7 +16,8 @@ import pytest
 import amrcheck as A
 from conftest import require
 
-CASES = sorted(f[:-5] for f in os.listdir(A.GOLDEN) if f.endswith(".json"))
+CASES = sorted(f[:-5] for f in os.listdir(A.GOLDEN)
+               if f.endswith(".json") and not f.startswith("report_"))
 
 
 @pytest.fixture(scope="module")
