@@ -441,6 +441,8 @@ def main():
                                "num_refine 4, 2 moving objects, timesteps %d-%d of 10"
                                % (a.warmup + 1, a.warmup + a.steps),
                    "schedule": a.schedule,
+                   "ghosts": ("same-level faces read in place"
+                              if os.environ.get("MINIAMR_FUSED", "1") != "0" else "plain"),
                    "avg_active_blocks": round(avg_blocks, 1),
                    "cell_var_updates": updates},
         "roofline": {"bound": "hbm",
@@ -448,8 +450,10 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
                      "traffic": traffic,
-                     "kernel": "k_stencil (27-point), %d launches, %.3f ms avg" % (
-                         launches, stencil_s / max(1, launches) * 1e3),
+                     "kernel": "k_stencil_fused + k_stencil (27-point; every stage but "
+                               "a timestep's last reads same-level faces in place), %d "
+                               "launches, %.3f ms avg" % (
+                                   launches, stencil_s / max(1, launches) * 1e3),
                      "step_level_frac": round(updates * 2 * ebytes / elapsed / 1e9 /
                                               HBM_PEAK_GBS, 4)},
         "device_ms": {"exchange": round(s1.device_ms[0] - s0.device_ms[0], 2),

@@ -151,18 +151,39 @@ int amr_dev_read_sums(amr_dev *dev, int row0, int n_rows, double *out);
 int amr_dev_stage(amr_dev *dev, int start, int num, int stage, int permute,
                   int sum_row);
 
+/* amr_dev_stage with same-level faces read in place (no counterpart in the
+ * reference's interface; the drop-in driver uses it for every stage of a
+ * timestep but the last).  Cells, checksums and every ghost value a stencil
+ * reads are those of amr_dev_stage.  The difference is where they come
+ * from: the ghost planes of same-level faces of blocks away from the
+ * domain boundary are not written -- the stencil reads those cells from
+ * the neighbour's interior, which stays unchanged because the stage writes
+ * its results to a second copy of every interior (records hold two
+ * unless the context was created with MINIAMR_FUSED=0 in the environment,
+ * which costs ~70% more pool memory at 16^3).  Those
+ * ghost planes are stale until the next amr_dev_stage or amr_dev_comm of
+ * the same variables; until then set_mesh, split, consolidate, stencil,
+ * download_block and save_state fail with AMR_ERR_STATE.  It is
+ * amr_dev_stage without two interiors, with permute != 0, while the tape
+ * records, and for stages of fewer than 1e8 cell updates (latency-bound;
+ * MINIAMR_FUSED=2 at creation fuses those too). */
+int amr_dev_stage_fused(amr_dev *dev, int start, int num, int stage, int permute,
+                        int sum_row);
+
 /* Wait for all enqueued work. */
 int amr_dev_sync(amr_dev *dev);
 
 /* Device time of the kernels enqueued since the last call, in
  * milliseconds, split as [0] exchange, [1] stencil, [2] checksum,
- * [3] split/consolidate.  Requires amr_dev_set_timing(dev, 1). */
+ * [3] split/consolidate.  Requires amr_dev_set_timing(dev, 1); level 2
+ * also times each exchange direction (amr_dev_read_exchange_timing), at
+ * the cost of two more events per direction and stage. */
 int amr_dev_set_timing(amr_dev *dev, int on);
 int amr_dev_read_timing(amr_dev *dev, double ms[4]);
 /* The exchange time of ms[0] split by direction (x, y, z), accumulated by
  * amr_dev_read_timing since the last call of this one.  For the end-of-run
  * report's per-direction rows (ref/profile.c:1085-1166); no counterpart
- * in the reference's call interface. */
+ * in the reference's call interface.  Zero unless timing level 2. */
 int amr_dev_read_exchange_timing(amr_dev *dev, double ms[3]);
 
 /* Bytes of device (and pinned host) memory the context holds, and in how

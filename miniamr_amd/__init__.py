@@ -35,6 +35,7 @@ ABI_SYMBOLS = [
     "amr_dev_check_sum", "amr_dev_split", "amr_dev_consolidate", "amr_dev_reserve_sums",
     "amr_dev_read_sums", "amr_dev_stage", "amr_dev_sync", "amr_dev_set_timing",
     "amr_dev_read_timing", "amr_dev_read_exchange_timing", "amr_dev_memory",
+    "amr_dev_stage_fused",
 ]
 
 
@@ -100,6 +101,7 @@ def load_library(path=LIB_PATH):
         "amr_dev_reserve_sums": (ip, [vp, ip]),
         "amr_dev_read_sums": (ip, [vp, ip, ip, dp]),
         "amr_dev_stage": (ip, [vp, ip, ip, ip, ip, ip]),
+        "amr_dev_stage_fused": (ip, [vp, ip, ip, ip, ip, ip]),
         "amr_dev_sync": (ip, [vp]),
         "amr_dev_set_timing": (ip, [vp, ip]),
         "amr_dev_read_timing": (ip, [vp, dp]),
@@ -383,6 +385,10 @@ class Device:
     def stage(self, start, num, stage=0, permute=False, sum_row=-1):
         self._check(self.lib.amr_dev_stage(self.h, start, num, stage, int(permute), sum_row),
                     "stage")
+
+    def stage_fused(self, start, num, stage=0, permute=False, sum_row=-1):
+        self._check(self.lib.amr_dev_stage_fused(self.h, start, num, stage, int(permute),
+                                                 sum_row), "stage_fused")
 
     def sync(self):
         self._check(self.lib.amr_dev_sync(self.h), "sync")

@@ -869,6 +869,8 @@ int amr_dev_state_bytes(const amr_dev *d, size_t *bytes)
 
 int amr_dev_save_state(amr_dev *d, void *dst)
 {
+   if (d && amr_require_fresh(d, 0, d->c.num_vars, "save_state"))
+      return AMR_ERR_STATE;
    size_t bytes = 0;
    if (!d || !dst || amr_dev_state_bytes(d, &bytes))
       return amr_set_error(d, AMR_ERR_ARG, "save_state: bad arguments");

@@ -39,8 +39,27 @@ struct Lay {
    int oF[6];               // face planes
    int oEXY, oEXZ, oEYZ;    // edge lines
    int oZI, oZX, oZY;       // z = 1 / z = nz copies
+   int oI;                  // the interior part in use (0, or the second
+                            // interior of a two-interior record)
    int64_t rec;             // elements per (var, slot), padded to 256 bytes
    int64_t var_stride;      // rec * max_blocks
+};
+
+// A face job of a stage that reads same-level faces in place
+// (amr_dev_stage_fused).  FJ_FULL fills the job's ghost cells as the plain
+// exchange does; FJ_EXT_ONLY (a same-level y or z face whose face interior
+// the stencil reads from the neighbour directly) fills only the cells
+// outside the face interior: the 27-point edge extension.  alt[] redirects
+// the job's reads of the source's own ghost rows that are themselves read
+// in place to the block they would have been copied from: y jobs alt[0] /
+// alt[1] for the source's x = 0 / x = nx+1 rows, z jobs the same plus
+// alt[2] / alt[3] for the y = 0 / y = ny+1 rows; -1 reads the source.
+enum { FJ_FULL = 0, FJ_EXT_ONLY = 1 };
+struct FusedJob {
+   amr_face_job jb;
+   int32_t mode;
+   int32_t alt[4];
+   int32_t pad;
 };
 
 // Recorded data operations of a context (include/amr_adjoint.h): each op
@@ -86,10 +105,31 @@ struct amr_dev {
    void *d_jobbuf = nullptr;        // split/merge job array
    size_t jobbuf_bytes = 0;
 
+   // Two-interior records (unless MINIAMR_FUSED=0 at creation): a second copy of the
+   // I and ZI parts, so a stage can read one and write the other.  par[v]
+   // is the copy that holds variable v's current interior; oI_par / oZI_par
+   // are the two copies' offsets in the record.
+   bool two_interiors = false;
+   int fused_mode = 1;              // MINIAMR_FUSED: 0 never, 1 large stages, 2 always
+   int oI_par[2] = {0, 0}, oZI_par[2] = {0, 0};
+   std::vector<uint8_t> par;
+
+   // The plan of amr_dev_stage_fused, derived from the plain plan by
+   // set_mesh: reduced job lists per direction, and per active block
+   // (sorted order) the slots its six faces are read from in place (-1:
+   // its own ghost plane).  stale[v]: variable v's in-place faces' ghost
+   // planes have not been written since the last plain exchange.
+   FusedJob *d_fjobs = nullptr;
+   int32_t fcount[3] = {0, 0, 0};
+   int fjobs_cap = 0;
+   int32_t *d_redir = nullptr;
+   std::vector<uint8_t> stale;
+
    Tape *tape = nullptr;            // recording of data operations, if any
    bool replaying = false;          // calls come from the tape: do not record
 
    bool timing = false;
+   bool timing_dirs = false;        // exchange spans per direction (timing level 2)
    double exchange_ms[3] = {0.0, 0.0, 0.0};   // per direction, read by read_timing
    std::vector<hipEvent_t> ev_free;
    struct Span { hipEvent_t a, b; int cat; };
@@ -174,6 +214,13 @@ template <typename T> __device__ __forceinline__ T ld_stream(gptr<T> base, uint3
    else
       return *p;
 }
+template <typename T> __device__ __forceinline__ T ld_stream64(gptr<T> base, int64_t i)
+{
+   if constexpr (sizeof(T) == 8)
+      return __builtin_nontemporal_load(base + i);
+   else
+      return base[i];
+}
 template <typename T> __device__ __forceinline__ void st_stream(gptr<T> base, uint32_t i, T v)
 {
    const gptr<T> p = (gptr<T>) ((gptr<char>) base + i * (uint32_t) sizeof(T));
@@ -191,7 +238,7 @@ __host__ __device__ __forceinline__ int home(const Lay &L, int i, int j, int k)
    const int xs = i != 0, ys = j != 0, zs = k != 0;
    switch (gi + gj + gk) {
    case 0:
-      return ((i - 1) * L.ny + (j - 1)) * L.nz + (k - 1);
+      return L.oI + ((i - 1) * L.ny + (j - 1)) * L.nz + (k - 1);
    case 1:
       if (gi) return L.oF[xs] + (j - 1) * L.nz + (k - 1);
       if (gj) return L.oF[2 + ys] + (i - 1) * L.nz + (k - 1);
@@ -252,9 +299,23 @@ int amr_merge_impl(amr_dev *d, const amr_merge_job *jobs, int n);
 void amr_tape_record(amr_dev *d, int kind, int a1 = 0, int a2 = 0, int a3 = 0, int a4 = 0);
 void amr_tape_free(amr_dev *d);
 
+// The layout of variables [start, start+num) with their current interior
+// copy (AMR_ERR_STATE, via amr_set_error, if the range mixes copies).
+int amr_lay(amr_dev *d, int start, int num, Lay *out);
+// The same layout with the other interior copy (where a stage writes).
+Lay amr_lay_other(const amr_dev *d, const Lay &L);
+
 // Launchers implemented per source file; all enqueue on d->stream.
 int amr_launch_exchange_dir(amr_dev *d, int dir, int start, int num);
+int amr_launch_exchange_fused_dir(amr_dev *d, int dir, int start, int num);
 int amr_launch_stencil(amr_dev *d, int start, int num, bool want_partials);
+// Stencil of a fused stage: interiors read from the current copy, written
+// to the other one (the variables' copy flips), same-level faces read in
+// place.
+int amr_launch_stencil_fused(amr_dev *d, int start, int num, bool want_partials);
+// AMR_ERR_STATE when any of variables [start, start+num) has stale ghost
+// planes (an amr_dev_stage_fused since its last plain exchange).
+int amr_require_fresh(amr_dev *d, int start, int num, const char *what);
 int amr_launch_block_sums(amr_dev *d, int start, int num);
 int amr_launch_reduce_sums(amr_dev *d, int start, int num, double *dst_row);
 int amr_launch_split(amr_dev *d, const amr_split_job *d_jobs, int n, void *scratch);

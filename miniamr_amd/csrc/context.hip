@@ -1,6 +1,7 @@
 // context.hip -- context lifetime, host<->device copies, mesh plan upload,
 // checksum table, fused stage entry point and timing for libminiamr_dev.so.
 #include <stdarg.h>
+#include <stdlib.h>
 #include <stdio.h>
 #include <string.h>
 #include <algorithm>
@@ -112,6 +113,18 @@ int amr_dev_create(const amr_dev_config *cfg, amr_dev **out)
    g.oZI = o; o += 2 * g.nx * g.ny;
    g.oZX = o; o += 4 * g.gy;
    g.oZY = o; o += 4 * g.nx;
+   g.oI = 0;
+   d->oI_par[0] = 0;
+   d->oZI_par[0] = d->oZI_par[1] = g.oZI;
+   const char *fz = getenv("MINIAMR_FUSED");
+   d->fused_mode = fz ? atoi(fz) : 1;
+   d->two_interiors = d->fused_mode != 0;
+   if (d->two_interiors) {                           // second I and ZI
+      d->oI_par[1] = o; o += g.nx * g.ny * g.nz;
+      d->oZI_par[1] = o; o += 2 * g.nx * g.ny;
+   }
+   d->par.assign(cfg->num_vars, 0);
+   d->stale.assign(cfg->num_vars, 0);
    d->elem_bytes = cfg->precision == AMR_PREC_F64 ? 8 : cfg->precision == AMR_PREC_F32 ? 4 : 2;
    int64_t per256 = 256 / d->elem_bytes;
    int64_t cells = (int64_t) g.gx * g.gy * g.gz;
@@ -151,6 +164,8 @@ int amr_dev_destroy(amr_dev *d)
    hipFree(d->d_stage);
    hipFree(d->d_scratch);
    hipFree(d->d_jobbuf);
+   hipFree(d->d_fjobs);
+   hipFree(d->d_redir);
    if (d->h_pinned)
       hipHostFree(d->h_pinned);
    if (d->stream && d->owns_stream)
@@ -181,6 +196,8 @@ int amr_dev_upload_block(amr_dev *d, int slot, const double *cells)
 
 int amr_dev_download_block(amr_dev *d, int slot, double *cells)
 {
+   if (d && amr_require_fresh(d, 0, d->c.num_vars, "download_block"))
+      return AMR_ERR_STATE;
    if (!d || !cells || slot < 0 || slot >= d->c.max_blocks)
       return amr_set_error(d, AMR_ERR_ARG, "download: slot %d out of range", slot);
    int rc = amr_launch_convert_out(d, slot);
@@ -193,6 +210,116 @@ int amr_dev_download_block(amr_dev *d, int slot, double *cells)
 }
 
 }  // extern "C"
+
+int amr_lay(amr_dev *d, int start, int num, Lay *out)
+{
+   *out = d->g;
+   if (num <= 0)
+      return AMR_OK;
+   const int p = d->par[start];
+   for (int v = start + 1; v < start + num; v++)
+      if (d->par[v] != p)
+         return amr_set_error(d, AMR_ERR_STATE,
+                              "variables %d and %d hold their cells in different copies",
+                              start, v);
+   out->oI = d->oI_par[p];
+   out->oZI = d->oZI_par[p];
+   return AMR_OK;
+}
+
+Lay amr_lay_other(const amr_dev *d, const Lay &L)
+{
+   Lay o = L;
+   const int p = L.oI == d->oI_par[1] && d->two_interiors ? 1 : 0;
+   o.oI = d->oI_par[1 - p];
+   o.oZI = d->oZI_par[1 - p];
+   return o;
+}
+
+// The plan of amr_dev_stage_fused.  A block's face is read in place when
+// its job is a same-level copy and the block has no boundary face (a
+// boundary fill with the 27-point extension reads the block's own ghost
+// planes as they were before the later phases of the stage, which an
+// in-place read cannot reproduce).  Such a face's job is dropped (x, and
+// y/z for the 7-point stencil) or reduced to its edge extension (y/z for
+// the 27-point stencil); every job that reads a source's ghost row that is
+// read in place is redirected to the block that row would be copied from.
+static int build_fused_plan(amr_dev *d, const int32_t *sorted, int n_active,
+                            const amr_face_job *jobs, const int32_t count[3])
+{
+   const int M = d->c.max_blocks;
+   std::vector<int32_t> same(6 * (size_t) M, -1);
+   std::vector<uint8_t> bc(M, 0);
+   const int nj = count[0] + count[1] + count[2];
+   for (int t = 0; t < nj; t++) {
+      const amr_face_job &j = jobs[t];
+      if (j.kind == AMR_FACE_BC)
+         bc[j.dst] = 1;
+      else if (j.kind == AMR_FACE_SAME)
+         same[6 * (size_t) j.dst + j.face] = j.src;
+   }
+   auto inplace = [&](int slot, int f) { return bc[slot] ? -1 : same[6 * (size_t) slot + f]; };
+   std::vector<FusedJob> fj;
+   fj.reserve(nj);
+   int32_t fc[3] = {0, 0, 0};
+   for (int t = 0, dir = 0; dir < 3; dir++)
+      for (int e = 0; e < count[dir]; e++, t++) {
+         const amr_face_job &j = jobs[t];
+         FusedJob f;
+         f.jb = j;
+         f.mode = FJ_FULL;
+         f.alt[0] = f.alt[1] = f.alt[2] = f.alt[3] = -1;
+         f.pad = 0;
+         if (j.kind == AMR_FACE_SAME && inplace(j.dst, j.face) >= 0) {
+            if (dir == 0 || d->g.stencil == 7)
+               continue;                    // nothing outside the face interior
+            f.mode = FJ_EXT_ONLY;
+         }
+         if (d->g.stencil == 27 && dir > 0 &&
+             (j.kind == AMR_FACE_SAME || j.kind == AMR_FACE_BC)) {
+            f.alt[0] = inplace(j.src, 0);
+            f.alt[1] = inplace(j.src, 1);
+            if (dir == 2) {
+               f.alt[2] = inplace(j.src, 2);
+               f.alt[3] = inplace(j.src, 3);
+            }
+         }
+         fj.push_back(f);
+         fc[dir]++;
+      }
+   std::vector<int32_t> redir(6 * (size_t) std::max(n_active, 1), -1);
+   for (int in = 0; in < n_active; in++)
+      for (int f = 0; f < 6; f++)
+         redir[6 * (size_t) in + f] = inplace(sorted[in], f);
+   if ((int) fj.size() > d->fjobs_cap) {
+      hipFree(d->d_fjobs);
+      d->d_fjobs = nullptr;
+      int cap = std::max((int) fj.size(), 2 * d->fjobs_cap);
+      HIP_TRY(d, hipMalloc(&d->d_fjobs, sizeof(FusedJob) * cap));
+      d->fjobs_cap = cap;
+   }
+   if (!fj.empty())
+      HIP_TRY(d, hipMemcpy(d->d_fjobs, fj.data(), sizeof(FusedJob) * fj.size(),
+                           hipMemcpyHostToDevice));
+   if (!d->d_redir)
+      HIP_TRY(d, hipMalloc(&d->d_redir, sizeof(int32_t) * 6 * (size_t) M));
+   HIP_TRY(d, hipMemcpy(d->d_redir, redir.data(), sizeof(int32_t) * redir.size(),
+                        hipMemcpyHostToDevice));
+   for (int i = 0; i < 3; i++)
+      d->fcount[i] = fc[i];
+   return AMR_OK;
+}
+
+int amr_require_fresh(amr_dev *d, int start, int num, const char *what)
+{
+   for (int v = start; v < start + num && v < (int) d->stale.size(); v++)
+      if (d->stale[v])
+         return amr_set_error(d, AMR_ERR_STATE,
+                              "%s: variable %d has ghost planes read in place by "
+                              "amr_dev_stage_fused; run amr_dev_stage (or amr_dev_comm) on it "
+                              "first", what, v);
+   return AMR_OK;
+}
 
 int amr_set_mesh_impl(amr_dev *d, const int32_t *sorted, int n_active,
                       const amr_face_job *jobs, const int32_t count[3])
@@ -207,6 +334,8 @@ int amr_set_mesh_impl(amr_dev *d, const int32_t *sorted, int n_active,
    }
    if (nj && !jobs)
       return amr_set_error(d, AMR_ERR_ARG, "set_mesh: jobs missing");
+   if (int rc = amr_require_fresh(d, 0, d->c.num_vars, "set_mesh"))
+      return rc;
    // Validate every job on the host: slots in range and faces/quarters legal,
    // so no kernel can address outside the pool.
    for (int i = 0; i < n_active; i++)
@@ -237,6 +366,8 @@ int amr_set_mesh_impl(amr_dev *d, const int32_t *sorted, int n_active,
    for (int i = 0; i < 3; i++)
       d->count[i] = count[i];
    d->plan_version++;
+   if (d->two_interiors)
+      return build_fused_plan(d, sorted, n_active, jobs, count);
    return AMR_OK;
 }
 
@@ -249,14 +380,24 @@ int amr_comm_impl(amr_dev *d, int start, int num, int stage, int permute)
                            start + num);
    if (!num)
       return AMR_OK;
+   // one span per direction when timing per direction, else one in all
+   if (!d->timing_dirs)
+      amr_span_begin(d, 0);
    for (int o = 0; o < 3; o++) {
       int dir = permute ? perm[stage % 6][o] : o;
-      amr_span_begin(d, kSpanExchangeX + dir);
+      if (d->timing_dirs)
+         amr_span_begin(d, kSpanExchangeX + dir);
       int rc = amr_launch_exchange_dir(d, dir, start, num);
-      amr_span_end(d);
+      if (d->timing_dirs)
+         amr_span_end(d);
       if (rc)
          return rc;
    }
+   if (!d->timing_dirs)
+      amr_span_end(d);
+   // every ghost cell of these variables is written again
+   for (int v = start; v < start + num; v++)
+      d->stale[v] = 0;
    return AMR_OK;
 }
 
@@ -264,6 +405,8 @@ int amr_stencil_impl(amr_dev *d, int var)
 {
    if (!d || var < 0 || var >= d->c.num_vars)
       return amr_set_error(d, AMR_ERR_ARG, "stencil: variable %d out of range", var);
+   if (int rc = amr_require_fresh(d, var, 1, "stencil"))
+      return rc;
    amr_span_begin(d, 1);
    int rc = amr_launch_stencil(d, var, 1, false);
    amr_span_end(d);
@@ -397,11 +540,15 @@ static int refine_data(amr_dev *d, const J *jobs, int n, bool split)
 
 int amr_split_impl(amr_dev *d, const amr_split_job *jobs, int n)
 {
+   if (d && amr_require_fresh(d, 0, d->c.num_vars, "split"))
+      return AMR_ERR_STATE;
    return refine_data(d, jobs, n, true);
 }
 
 int amr_merge_impl(amr_dev *d, const amr_merge_job *jobs, int n)
 {
+   if (d && amr_require_fresh(d, 0, d->c.num_vars, "consolidate"))
+      return AMR_ERR_STATE;
    return refine_data(d, jobs, n, false);
 }
 
@@ -499,6 +646,55 @@ int amr_dev_stage(amr_dev *d, int start, int num, int stage, int permute, int su
    return rc;
 }
 
+int amr_dev_stage_fused(amr_dev *d, int start, int num, int stage, int permute, int sum_row)
+{
+   if (!d)
+      return AMR_ERR_ARG;
+   // the plain stage where in-place reads cannot stand in for it, and for
+   // stages below ~1e8 cell updates: those are bound by launch latency,
+   // where the fused kernels' longer set-up costs more than the ghost planes
+   // they save (FP32 runs of 8^3-16^3 blocks with 8-40 variables measured
+   // 1-7% slower fused; the 16^3, 40-variable bandwidth-scale run 15-18%
+   // faster)
+   const double cells = (double) d->n_active * num * d->g.nx * d->g.ny * d->g.nz;
+   if (!d->two_interiors || permute || (d->tape && d->tape->on) || d->replaying ||
+       (d->fused_mode == 1 && cells < 1.0e8))
+      return amr_dev_stage(d, start, num, stage, permute, sum_row);
+   if (start < 0 || num < 0 || start + num > d->c.num_vars)
+      return amr_set_error(d, AMR_ERR_ARG, "stage: variables out of range");
+   if (sum_row >= d->sum_rows)
+      return amr_set_error(d, AMR_ERR_ARG, "stage: checksum row %d not reserved", sum_row);
+   if (!num)
+      return AMR_OK;
+   if (!d->timing_dirs)
+      amr_span_begin(d, 0);
+   for (int dir = 0; dir < 3; dir++) {
+      if (d->timing_dirs)
+         amr_span_begin(d, kSpanExchangeX + dir);
+      int rc = amr_launch_exchange_fused_dir(d, dir, start, num);
+      if (d->timing_dirs)
+         amr_span_end(d);
+      if (rc)
+         return rc;
+   }
+   if (!d->timing_dirs)
+      amr_span_end(d);
+   amr_span_begin(d, 1);
+   int rc = amr_launch_stencil_fused(d, start, num, sum_row >= 0);
+   amr_span_end(d);
+   if (rc)
+      return rc;
+   for (int v = start; v < start + num; v++)
+      d->stale[v] = 1;
+   if (sum_row < 0)
+      return AMR_OK;
+   amr_span_begin(d, 2);
+   rc = amr_launch_reduce_sums(d, start, num,
+                               d->d_sums + (size_t) sum_row * d->c.num_vars + start);
+   amr_span_end(d);
+   return rc;
+}
+
 int amr_dev_sync(amr_dev *d)
 {
    if (!d)
@@ -512,6 +708,7 @@ int amr_dev_set_timing(amr_dev *d, int on)
    if (!d)
       return AMR_ERR_ARG;
    d->timing = on != 0;
+   d->timing_dirs = on >= 2;
    return AMR_OK;
 }
 

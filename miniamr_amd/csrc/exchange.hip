@@ -40,6 +40,17 @@ __global__ __launch_bounds__(NT) void k_exchange(T *__restrict__ pool,
    exchange_job<T, KV>(pool, jb, L, v0, min(KV, var_end - v0), threadIdx.x, NT);
 }
 
+// The fused stage's jobs (FusedJob), same launch shape.
+template <typename T, int KV, int NT>
+__global__ __launch_bounds__(NT) void k_exchange_fused(T *__restrict__ pool,
+                                                       const FusedJob *__restrict__ jobs, Lay L,
+                                                       int var0, int var_end)
+{
+   const FusedJob fj = jobs[blockIdx.x];
+   const int v0 = var0 + blockIdx.y * KV;
+   exchange_job_fused<T, KV>(pool, fj, L, v0, min(KV, var_end - v0), threadIdx.x, NT);
+}
+
 // Variables and threads per workgroup.  Default 8 x 256 (measured best for
 // FP64 and FP32 on the 16^3 workload: 8 x 128 was 3% slower in FP32, 16
 // variables 1-3% slower); MINIAMR_EXCHANGE ("vars,threads" with vars
@@ -69,8 +80,11 @@ template <typename T, int KV, int NT>
 int launch_kv(amr_dev *d, int off, int n, int start, int num)
 {
    dim3 grid(n, (num + KV - 1) / KV);
+   Lay L;
+   if (int rc = amr_lay(d, start, num, &L))
+      return rc;
    hipLaunchKernelGGL((k_exchange<T, KV, NT>), grid, dim3(NT), 0, d->stream, (T *) d->pool,
-                      d->d_jobs + off, d->g, start, start + num);
+                      d->d_jobs + off, L, start, start + num);
    HIP_TRY(d, hipGetLastError());
    return AMR_OK;
 }
@@ -92,7 +106,33 @@ int launch(amr_dev *d, int dir, int start, int num)
                      : launch_kv<T, 8, 128>(d, off, n, start, num);
 }
 
+template <typename T>
+int launch_fused(amr_dev *d, int dir, int start, int num)
+{
+   const int off = dir == 0 ? 0 : dir == 1 ? d->fcount[0] : d->fcount[0] + d->fcount[1];
+   const int n = d->fcount[dir];
+   if (!n || !num)
+      return AMR_OK;
+   Lay L;
+   if (int rc = amr_lay(d, start, num, &L))
+      return rc;
+   dim3 grid(n, (num + kVars - 1) / kVars);
+   hipLaunchKernelGGL((k_exchange_fused<T, kVars, 256>), grid, dim3(256), 0, d->stream,
+                      (T *) d->pool, d->d_fjobs + off, L, start, start + num);
+   HIP_TRY(d, hipGetLastError());
+   return AMR_OK;
+}
+
 }  // namespace
+
+int amr_launch_exchange_fused_dir(amr_dev *d, int dir, int start, int num)
+{
+   switch (d->c.precision) {
+   case AMR_PREC_F64: return launch_fused<double>(d, dir, start, num);
+   case AMR_PREC_F32: return launch_fused<float>(d, dir, start, num);
+   default: return launch_fused<uint16_t>(d, dir, start, num);
+   }
+}
 
 int amr_launch_exchange_dir(amr_dev *d, int dir, int start, int num)
 {

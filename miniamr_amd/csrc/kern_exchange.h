@@ -156,4 +156,106 @@ __device__ __forceinline__ void exchange_job(T *__restrict__ pool, const amr_fac
    }
 }
 
+// One face job of a fused stage (FusedJob).  Prolongation and restriction
+// are the plain fill.  Same-level and boundary jobs fill their cells as the
+// plain fill does, except that an FJ_EXT_ONLY job fills only the cells
+// outside the face interior (two rows for y, the ring of the plane for z),
+// and a read of a source ghost row that is read in place (alt[] >= 0) goes
+// to the cell that row would hold: the x = 0 row of the source is the
+// interior x = nx plane of the block alt[0], and so on.  The source block
+// then varies per cell, so reads use 64-bit offsets from the variable's
+// base.
+template <typename T, int KV = kVars>
+__device__ __forceinline__ void exchange_job_fused(T *__restrict__ pool, const FusedJob &fj,
+                                                   const Lay &L, int v0, int nv, int tid,
+                                                   int nthr)
+{
+   const amr_face_job &jb = fj.jb;
+   if (jb.kind == AMR_FACE_PROLONG || jb.kind == AMR_FACE_RESTRICT) {
+      exchange_job<T, KV>(pool, jb, L, v0, nv, tid, nthr);
+      return;
+   }
+   const int dir = jb.face >> 1, hi = jb.face & 1;
+   const int n = dir == 0 ? L.nx : dir == 1 ? L.ny : L.nz;
+   const int na = dir == 0 ? L.ny : L.nx;
+   const int nb = dir == 2 ? L.ny : L.nz;
+   const int ghost = hi ? n + 1 : 0;
+   int ext_a = 0, ext_b = 0;
+   if (L.stencil == 27) {
+      if (jb.kind == AMR_FACE_BC) ext_a = ext_b = 1;
+      else if (dir == 1) ext_a = 1;
+      else if (dir == 2) ext_a = ext_b = 1;
+   }
+   const int a0 = 1 - ext_a, b0 = 1 - ext_b;
+   const int wa = na + 2 * ext_a, wb = nb + 2 * ext_b;
+   const bool ext_only = fj.mode == FJ_EXT_ONLY;
+   // ext-only: rows ai = 0 and wa-1 (all bi), then, with ext_b, columns
+   // bi = 0 and wb-1 of the rows between
+   const int ncell = ext_only ? 2 * wb + (ext_b ? 2 * (wa - 2) : 0) : wa * wb;
+   const int sp = jb.kind == AMR_FACE_BC ? (hi ? n : 1) : (hi ? 1 : n);
+   const int64_t vs = L.var_stride;
+   const gptr<T> base = global_view(pool) + (int64_t) v0 * vs;
+   const int64_t dst0 = (int64_t) jb.dst * L.rec;
+   const int G = var_groups<KV>(ncell, nthr), cw = nthr / G;
+   const int g = __builtin_amdgcn_readfirstlane(tid / cw), c0 = tid - g * cw;
+   for (int t = c0; t < ncell; t += cw) {
+      int ai, bi;
+      if (!ext_only) {
+         ai = t / wb;
+         bi = t - ai * wb;
+      } else if (t < 2 * wb) {
+         ai = t < wb ? 0 : wa - 1;
+         bi = t < wb ? t : t - wb;
+      } else {
+         const int u = t - 2 * wb;
+         bi = u < wa - 2 ? 0 : wb - 1;
+         ai = 1 + (u < wa - 2 ? u : u - (wa - 2));
+      }
+      const int a = a0 + ai, b = b0 + bi;
+      int i, j, k, si, sj, sk;
+      canon(dir, sp, a, b, si, sj, sk);
+      canon(dir, ghost, a, b, i, j, k);
+      int slot = jb.src;
+      int so = dir == 2 ? zcopy(L, si, sj, sk) : home(L, si, sj, sk);
+      if (dir == 1) {
+         if (si == 0 && fj.alt[0] >= 0) {
+            slot = fj.alt[0];
+            so = home(L, L.nx, sj, sk);
+         } else if (si == L.nx + 1 && fj.alt[1] >= 0) {
+            slot = fj.alt[1];
+            so = home(L, 1, sj, sk);
+         }
+      } else if (dir == 2) {
+         const bool xin = si >= 1 && si <= L.nx, yin = sj >= 1 && sj <= L.ny;
+         if (si == 0 && yin && fj.alt[0] >= 0) {
+            slot = fj.alt[0];
+            so = zcopy(L, L.nx, sj, sk);
+         } else if (si == L.nx + 1 && yin && fj.alt[1] >= 0) {
+            slot = fj.alt[1];
+            so = zcopy(L, 1, sj, sk);
+         } else if (sj == 0 && xin && fj.alt[2] >= 0) {
+            slot = fj.alt[2];
+            so = zcopy(L, si, L.ny, sk);
+         } else if (sj == L.ny + 1 && xin && fj.alt[3] >= 0) {
+            slot = fj.alt[3];
+            so = zcopy(L, si, 1, sk);
+         }
+      }
+      const int64_t src = (int64_t) slot * L.rec + so;
+      const int64_t dh = dst0 + home(L, i, j, k);
+      const int64_t dc = (k == 1 || k == L.nz) ? dst0 + zcopy(L, i, j, k) : -1;
+      T v[KV];
+#pragma unroll
+      for (int q = 0; q < KV; q++)
+         if (q < nv && (q & (G - 1)) == g)
+            v[q] = base[q * vs + src];
+#pragma unroll
+      for (int q = 0; q < KV; q++)
+         if (q < nv && (q & (G - 1)) == g) {
+            base[q * vs + dh] = v[q];
+            if (dc >= 0) base[q * vs + dc] = v[q];
+         }
+   }
+}
+
 }  // namespace amrk

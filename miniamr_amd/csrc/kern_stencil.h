@@ -3,6 +3,8 @@
 // persistent kernel (batch.hip).  See stencil.hip for the algorithm.
 #pragma once
 
+#include <type_traits>
+
 #include "amr_internal.h"
 
 namespace amrk {
@@ -83,6 +85,17 @@ __device__ __forceinline__ A divide(A w)
    return (A) __fmaf_rn(e, r, q);
 }
 
+// A plane element's load: 32-bit offset from a record, or (fused stages)
+// 64-bit offset from a variable's base.
+template <typename T> __device__ __forceinline__ T load_at(gptr<T> b, int o)
+{
+   return ld_stream(b, (uint32_t) o);
+}
+template <typename T> __device__ __forceinline__ T load_at(gptr<T> b, int64_t o)
+{
+   return ld_stream64(b, o);
+}
+
 // One (block, up to VPW variables) item, computed by NT threads: the whole
 // workgroup (NT == kThreads) or one wavefront (NT == 64).  rec points at
 // the block's record of the first variable; `ring` is 3*VPW planes of LDS
@@ -93,22 +106,45 @@ __device__ __forceinline__ A divide(A w)
 // compile time (the LDS neighbour offsets and the column decomposition
 // then become constants); 0 reads it from L.  D planes of loads are kept
 // in flight (small blocks can afford all of theirs).
+//
+// FUSED (a stage that reads same-level faces in place): the block's slot
+// bslot and redir[6], the slots its six faces are read from (-1: its own ghost
+// plane).  The face interior of a redirected face is read from the
+// neighbour's interior (its z-plane copy for z faces) in the copy L
+// describes, and the outputs go to the copy Lout describes, so no
+// neighbour's cells change while they are read.  Plane offsets are then
+// 64-bit offsets from the variable's base, the source block varying per
+// element.
 template <typename T, int ST, int NCOL, int NPER, int VPW, int NT = kThreads, int NZ = 0,
-          int D = kDepth>
+          int D = kDepth, bool FUSED = false>
 __device__ __forceinline__ void stencil_block(T *__restrict__ rec, const Lay &L, int nv,
                                               typename Acc<T>::type *ring, double *red,
-                                              double *__restrict__ partials, int part_stride)
+                                              double *__restrict__ partials, int part_stride,
+                                              const Lay *Lout = nullptr, int bslot = 0,
+                                              const int32_t *__restrict__ redir = nullptr)
 {
+   // interior cells are read with L and written with Lo: the same interior
+   // copy (in place), or the record's other copy
+   const Lay &Lo = Lout ? *Lout : L;
+   typedef typename std::conditional<FUSED, int64_t, int>::type Off;
    typedef typename Acc<T>::type A;
    const int nz = NZ ? NZ : L.nz, gz = nz + 2;
    const int plane = L.gy * gz, ncol = L.ny * nz, nplanes = L.nx + 2;
    const int64_t vs = L.var_stride;
    const int tid = NT == kThreads ? (int) threadIdx.x : (int) (threadIdx.x & (NT - 1));
    // the item's records, one per variable, as scalar bases
-   gptr<T> ub[VPW];
+   gptr<T> ub[VPW], lb[VPW];   // output bases; load bases (FUSED: the variables')
 #pragma unroll
-   for (int u = 0; u < VPW; u++)
+   for (int u = 0; u < VPW; u++) {
       ub[u] = uniform_base(global_view(rec) + u * vs);
+      lb[u] = FUSED ? uniform_base(global_view(rec) - (int64_t) bslot * L.rec + u * vs) : ub[u];
+   }
+   int rdr[6] = {-1, -1, -1, -1, -1, -1};
+   if constexpr (FUSED) {
+#pragma unroll
+      for (int f = 0; f < 6; f++)
+         rdr[f] = __builtin_amdgcn_readfirstlane(redir[f]);
+   }
 
    // This thread's plane elements r = tid + q*NT: the ny*nz interior cells
    // first, in record order (a wavefront's loads then cover whole cache
@@ -116,7 +152,8 @@ __device__ __forceinline__ void stencil_block(T *__restrict__ rec, const Lay &L,
    // and k = nz+1.  pos is the element's place in an LDS plane; the record
    // offsets are those of plane 0, plane nx+1 and interior planes p, where
    // the offset is affine in p.
-   int off0[NPER], offN[NPER], off1[NPER], stp[NPER], pos[NPER];
+   Off off0[NPER], offN[NPER], off1[NPER];
+   int stp[NPER], pos[NPER];
 #pragma unroll
    for (int q = 0; q < NPER; q++) {
       const int r = min(tid + q * NT, plane - 1);
@@ -136,10 +173,42 @@ __device__ __forceinline__ void stencil_block(T *__restrict__ rec, const Lay &L,
          }
       }
       pos[q] = jj * gz + kk;
-      off0[q] = home(L, 0, jj, kk);
-      offN[q] = home(L, L.nx + 1, jj, kk);
-      off1[q] = home(L, 1, jj, kk);
-      stp[q] = L.nx > 1 ? home(L, 2, jj, kk) - off1[q] : 0;
+      if constexpr (!FUSED) {
+         off0[q] = home(L, 0, jj, kk);
+         offN[q] = home(L, L.nx + 1, jj, kk);
+         off1[q] = home(L, 1, jj, kk);
+         stp[q] = L.nx > 1 ? home(L, 2, jj, kk) - off1[q] : 0;
+      } else {
+         const int64_t own = (int64_t) bslot * L.rec;
+         const bool jin = jj >= 1 && jj <= L.ny, kin = kk >= 1 && kk <= nz;
+         off0[q] = jin && kin && rdr[0] >= 0 ? (int64_t) rdr[0] * L.rec + home(L, L.nx, jj, kk)
+                                             : own + home(L, 0, jj, kk);
+         offN[q] = jin && kin && rdr[1] >= 0 ? (int64_t) rdr[1] * L.rec + home(L, 1, jj, kk)
+                                             : own + home(L, L.nx + 1, jj, kk);
+         // interior planes: a y / z face row from the neighbour's interior
+         // (z: its z-plane copy), else the block's own record
+         int o1 = home(L, 1, jj, kk), o2 = L.nx > 1 ? home(L, 2, jj, kk) : o1;
+         int64_t b = own;
+         if (kin && jj == 0 && rdr[2] >= 0) {
+            b = (int64_t) rdr[2] * L.rec;
+            o1 = home(L, 1, L.ny, kk);
+            o2 = L.nx > 1 ? home(L, 2, L.ny, kk) : o1;
+         } else if (kin && jj == L.ny + 1 && rdr[3] >= 0) {
+            b = (int64_t) rdr[3] * L.rec;
+            o1 = home(L, 1, 1, kk);
+            o2 = L.nx > 1 ? home(L, 2, 1, kk) : o1;
+         } else if (jin && kk == 0 && rdr[4] >= 0) {
+            b = (int64_t) rdr[4] * L.rec;
+            o1 = zcopy(L, 1, jj, nz);
+            o2 = L.nx > 1 ? zcopy(L, 2, jj, nz) : o1;
+         } else if (jin && kk == nz + 1 && rdr[5] >= 0) {
+            b = (int64_t) rdr[5] * L.rec;
+            o1 = zcopy(L, 1, jj, 1);
+            o2 = L.nx > 1 ? zcopy(L, 2, jj, 1) : o1;
+         }
+         off1[q] = b + o1;
+         stp[q] = o2 - o1;
+      }
    }
    // Per output column col = (j-1)*nz + (k-1): its index in an LDS plane
    // (-1: no such column) and the offset of its z-plane copy at x = 1 (-1:
@@ -155,17 +224,17 @@ __device__ __forceinline__ void stencil_block(T *__restrict__ rec, const Lay &L,
          const int j = col / nz + 1, k = col - (j - 1) * nz + 1;
          cpl[q] = j * gz + k;
          if (k == 1 || k == nz)
-            zc1[q] = zcopy(L, 1, j, k);
+            zc1[q] = zcopy(Lo, 1, j, k);
       }
    }
    T pre[D][VPW][NPER];
 #define PREFETCH(slot, p)                                                  \
    _Pragma("unroll") for (int q = 0; q < NPER; q++) {                      \
       if (tid + q * NT < plane) {                                          \
-         const int o = (p) == 0 ? off0[q] : (p) == L.nx + 1 ? offN[q]       \
+         const Off o = (p) == 0 ? off0[q] : (p) == L.nx + 1 ? offN[q]       \
                                          : off1[q] + ((p) - 1) * stp[q];    \
          _Pragma("unroll") for (int u = 0; u < VPW; u++)                   \
-            if (u < nv) pre[slot][u][q] = ld_stream(ub[u], o);             \
+            if (u < nv) pre[slot][u][q] = load_at(lb[u], o);               \
       }                                                                    \
    }
 
@@ -235,7 +304,7 @@ __device__ __forceinline__ void stencil_block(T *__restrict__ rec, const Lay &L,
                }
                if (out_now) {
                   const T out = from_acc<T>(w);
-                  st_stream(ub[u], (i - 1) * ncol + tid + q * NT, out);
+                  st_stream(ub[u], Lo.oI + (i - 1) * ncol + tid + q * NT, out);
                   if (zc1[q] >= 0)
                      st_stream(ub[u], zc1[q] + (i - 1) * L.ny, out);
                   if (partials)

@@ -29,7 +29,7 @@ __global__ void k_split_snap(const T *__restrict__ pool, const amr_split_job *__
    T *s = scratch + ((int64_t) blockIdx.x * nvars + var) * hc;
    for (int t = threadIdx.x; t < hc; t += blockDim.x) {
       const int i = t / (hy * hz), r = t - i * hy * hz, jj = r / hz, k = r - jj * hz;
-      s[t] = p[((i + i1) * g.ny + (jj + j1)) * g.nz + (k + k1)];
+      s[t] = p[g.oI + ((i + i1) * g.ny + (jj + j1)) * g.nz + (k + k1)];
    }
 }
 
@@ -60,7 +60,7 @@ __global__ void k_merge_sum(const T *__restrict__ pool, const amr_merge_job *__r
    const amr_merge_job j = jobs[blockIdx.x];
    const int var = blockIdx.y;
    const int hy = g.ny / 2, hz = g.nz / 2, hc = (g.nx / 2) * hy * hz;
-   const T *c = pool + (int64_t) var * g.var_stride + (int64_t) j.child * g.rec;
+   const T *c = pool + (int64_t) var * g.var_stride + (int64_t) j.child * g.rec + g.oI;
    T *s = scratch + ((int64_t) blockIdx.x * nvars + var) * hc;
    const int sx = g.ny * g.nz, sy = g.nz;
    for (int t = threadIdx.x; t < hc; t += blockDim.x) {
@@ -98,11 +98,14 @@ template <typename T>
 int split(amr_dev *d, const amr_split_job *jobs, int n, void *scratch)
 {
    dim3 grid(n, d->c.num_vars);
+   Lay L;
+   if (int rc = amr_lay(d, 0, d->c.num_vars, &L))
+      return rc;
    hipLaunchKernelGGL(k_split_snap<T>, grid, dim3(256), 0, d->stream, (const T *) d->pool, jobs,
-                      (T *) scratch, d->g, d->c.num_vars);
+                      (T *) scratch, L, d->c.num_vars);
    HIP_TRY(d, hipGetLastError());
    hipLaunchKernelGGL(k_split_write<T>, grid, dim3(256), 0, d->stream, (T *) d->pool, jobs,
-                      (const T *) scratch, d->g, d->c.num_vars);
+                      (const T *) scratch, L, d->c.num_vars);
    HIP_TRY(d, hipGetLastError());
    return AMR_OK;
 }
@@ -111,11 +114,14 @@ template <typename T>
 int merge(amr_dev *d, const amr_merge_job *jobs, int n, void *scratch)
 {
    dim3 grid(n, d->c.num_vars);
+   Lay L;
+   if (int rc = amr_lay(d, 0, d->c.num_vars, &L))
+      return rc;
    hipLaunchKernelGGL(k_merge_sum<T>, grid, dim3(256), 0, d->stream, (const T *) d->pool, jobs,
-                      (T *) scratch, d->g, d->c.num_vars);
+                      (T *) scratch, L, d->c.num_vars);
    HIP_TRY(d, hipGetLastError());
    hipLaunchKernelGGL(k_merge_write<T>, grid, dim3(256), 0, d->stream, (T *) d->pool, jobs,
-                      (const T *) scratch, d->g, d->c.num_vars);
+                      (const T *) scratch, L, d->c.num_vars);
    HIP_TRY(d, hipGetLastError());
    return AMR_OK;
 }

@@ -51,6 +51,29 @@ __global__ __launch_bounds__(kThreads) void k_stencil(T *__restrict__ pool,
                                          part_stride);
 }
 
+// The stencil of a fused stage: interiors read from the copy L describes,
+// written to the copy Lo describes, same-level faces read in place through
+// redir (six slots per active block, sorted order).
+template <typename T, int ST, int NC, int NP, int V, int NZ>
+__global__ __launch_bounds__(kThreads) void k_stencil_fused(T *__restrict__ pool,
+                                                             const int32_t *__restrict__ sorted,
+                                                             const int32_t *__restrict__ redir,
+                                                             Lay L, Lay Lo, int var0, int var_end,
+                                                             double *__restrict__ partials,
+                                                             int part_stride)
+{
+   typedef typename Acc<T>::type A;
+   extern __shared__ __align__(16) unsigned char smem[];
+   __shared__ double red[kThreads / 64];
+   const int in = blockIdx.x, v0 = var0 + blockIdx.y * V;
+   const int nv = min(V, var_end - v0);
+   const int slot = sorted[in];
+   T *rec = pool + (int64_t) v0 * L.var_stride + (int64_t) slot * L.rec;
+   stencil_block<T, ST, NC, NP, V, kThreads, NZ, kDepth, true>(
+      rec, L, nv, (A *) smem, red, partials ? partials + (int64_t) v0 * part_stride + in : nullptr,
+      part_stride, &Lo, slot, redir + 6 * (int64_t) in);
+}
+
 // Per-block interior sums of variables [var0, var0 + gridDim.y).
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_block_sums(const T *__restrict__ pool,
@@ -61,7 +84,7 @@ __global__ __launch_bounds__(kThreads) void k_block_sums(const T *__restrict__ p
 {
    __shared__ double red[kThreads / 64];
    const int in = blockIdx.x, var = var0 + blockIdx.y;
-   const T *rec = pool + (int64_t) var * L.var_stride + (int64_t) sorted[in] * L.rec;
+   const T *rec = pool + (int64_t) var * L.var_stride + (int64_t) sorted[in] * L.rec + L.oI;
    const int ncell = L.nx * L.ny * L.nz;
    double bsum = 0.0;
    for (int t = threadIdx.x; t < ncell; t += kThreads)
@@ -146,7 +169,9 @@ static int depth6()
 template <typename T, int ST, int V>
 int launch_st_v(amr_dev *d, int start, int num, double *parts)
 {
-   const Lay &L = d->g;
+   Lay L;
+   if (int rc = amr_lay(d, start, num, &L))
+      return rc;
    const int ncol = L.ny * L.nz, plane = L.gy * L.gz;
    dim3 grid(d->n_active, (num + V - 1) / V);
    size_t lds = sizeof(typename Acc<T>::type) * 3 * V * (size_t) plane;
@@ -198,13 +223,51 @@ int launch_stencil(amr_dev *d, int start, int num, bool want_partials)
                             : launch_st<T, 27>(d, start, num, parts);
 }
 
+template <typename T, int ST>
+int launch_fused(amr_dev *d, int start, int num, double *parts)
+{
+   constexpr int V = Vpw<T>::v;
+   Lay L;
+   if (int rc = amr_lay(d, start, num, &L))
+      return rc;
+   const Lay Lo = amr_lay_other(d, L);
+   const int ncol = L.ny * L.nz, plane = L.gy * L.gz;
+   dim3 grid(d->n_active, (num + V - 1) / V);
+   size_t lds = sizeof(typename Acc<T>::type) * 3 * V * (size_t) plane;
+#define GO(NC, NP, NZ)                                                                        \
+   hipLaunchKernelGGL((k_stencil_fused<T, ST, NC, NP, V, NZ>), grid, dim3(kThreads), lds,        \
+                      d->stream, (T *) d->pool, d->d_sorted, d->d_redir, L, Lo, start,          \
+                      start + num, parts, d->c.max_blocks)
+   if (ncol <= kThreads && plane <= 2 * kThreads) {
+      if (L.nz == 16) GO(1, 2, 16);
+      else if (L.nz == 12) GO(1, 2, 12);
+      else if (L.nz == 8) GO(1, 2, 8);
+      else GO(1, 2, 0);
+   } else if (ncol <= 2 * kThreads && plane <= 4 * kThreads)
+      GO(2, 4, 0);
+   else if (ncol <= kCols * kThreads && plane <= kPer * kThreads)
+      GO(kCols, kPer, 0);
+   else
+      return amr_set_error(d, AMR_ERR_ARG, "block face %dx%d too large for the stencil kernel",
+                           L.ny, L.nz);
+#undef GO
+   HIP_TRY(d, hipGetLastError());
+   // the variables' cells now live in the other copy
+   for (int v = start; v < start + num; v++)
+      d->par[v] ^= 1;
+   return AMR_OK;
+}
+
 template <typename T>
 int launch_block_sums(amr_dev *d, int start, int num)
 {
    if (!d->n_active || !num)
       return AMR_OK;
+   Lay L;
+   if (int rc = amr_lay(d, start, num, &L))
+      return rc;
    hipLaunchKernelGGL(k_block_sums<T>, dim3(d->n_active, num), dim3(kThreads), 0, d->stream,
-                      (const T *) d->pool, d->d_sorted, d->g, start, d->d_partials,
+                      (const T *) d->pool, d->d_sorted, L, start, d->d_partials,
                       d->c.max_blocks);
    HIP_TRY(d, hipGetLastError());
    return AMR_OK;
@@ -215,17 +278,39 @@ int launch_convert(amr_dev *d, int slot, bool in)
 {
    int cells = d->g.gx * d->g.gy * d->g.gz;
    dim3 grid((cells + 255) / 256, d->c.num_vars);
+   Lay L;
+   if (int rc = amr_lay(d, 0, d->c.num_vars, &L))
+      return rc;
    if (in)
       hipLaunchKernelGGL(k_convert_in<T>, grid, dim3(256), 0, d->stream, (T *) d->pool,
-                         (const double *) d->d_stage, d->g, slot, cells);
+                         (const double *) d->d_stage, L, slot, cells);
    else
       hipLaunchKernelGGL(k_convert_out<T>, grid, dim3(256), 0, d->stream,
-                         (const T *) d->pool, d->d_stage, d->g, slot, cells);
+                         (const T *) d->pool, d->d_stage, L, slot, cells);
    HIP_TRY(d, hipGetLastError());
    return AMR_OK;
 }
 
 }  // namespace
+
+int amr_launch_stencil_fused(amr_dev *d, int start, int num, bool want_partials)
+{
+   if (!d->n_active || !num) {
+      for (int v = start; v < start + num; v++)
+         d->par[v] ^= 1;   // nothing to compute: the stage still flips the copy
+      return AMR_OK;
+   }
+   double *parts = want_partials ? d->d_partials : nullptr;
+   const bool s7 = d->g.stencil == 7;
+   switch (d->c.precision) {
+   case AMR_PREC_F64: return s7 ? launch_fused<double, 7>(d, start, num, parts)
+                                : launch_fused<double, 27>(d, start, num, parts);
+   case AMR_PREC_F32: return s7 ? launch_fused<float, 7>(d, start, num, parts)
+                                : launch_fused<float, 27>(d, start, num, parts);
+   default: return s7 ? launch_fused<uint16_t, 7>(d, start, num, parts)
+                      : launch_fused<uint16_t, 27>(d, start, num, parts);
+   }
+}
 
 int amr_launch_stencil(amr_dev *d, int start, int num, bool want_partials)
 {

@@ -86,6 +86,23 @@ double amr_calc_time_step(amr_run *R)
    return delta > 0.0 ? 1.0 / delta : 1.0;
 }
 
+/* One stage of variables [start, start+number).  Every stage of a
+ * timestep but the last reads same-level faces in place (amr_dev_stage_fused:
+ * the same cells and checksums); the last is a plain stage, so that at the
+ * end of every timestep -- where refinement, checksums and dumps look at
+ * the cells -- every ghost plane holds what the reference's would. */
+static void enqueue_stage(amr_run *R, int start, int number, int stage, int comm_stage,
+                          int row)
+{
+   const amr_params *P = &R->P;
+   if (stage < P->stages_per_ts - 1)
+      amr_dev_check(R, amr_dev_stage_fused(R->dev, start, number, comm_stage, P->permute, row),
+                    "stage");
+   else
+      amr_dev_check(R, amr_dev_stage(R->dev, start, number, comm_stage, P->permute, row),
+                    "stage");
+}
+
 /* Enqueue one timestep; checksum event e of the timestep goes to row e. */
 static void enqueue_timestep(amr_run *R, int comm_stage0)
 {
@@ -97,8 +114,7 @@ static void enqueue_timestep(amr_run *R, int comm_stage0)
                       ? stage / P->checksum_freq : -1;
          for (int start = 0; start < V; start += P->comm_vars) {
             int number = (start + P->comm_vars > V) ? V - start : P->comm_vars;
-            amr_dev_check(R, amr_dev_stage(R->dev, start, number, comm_stage0 + stage,
-                                           P->permute, row), "stage");
+            enqueue_stage(R, start, number, stage, comm_stage0 + stage, row);
             R->loop.stage_calls++;
             amr_count_comm(R, 1);
          }
@@ -123,8 +139,7 @@ static void enqueue_timestep(amr_run *R, int comm_stage0)
          for (int stage = 0; stage < P->stages_per_ts; stage++) {
             int row = (P->checksum_freq && !(stage % P->checksum_freq))
                          ? stage / P->checksum_freq : -1;
-            amr_dev_check(R, amr_dev_stage(R->dev, start, number, comm_stage0 + stage,
-                                           P->permute, row), "stage");
+            enqueue_stage(R, start, number, stage, comm_stage0 + stage, row);
             R->loop.stage_calls++;
             amr_count_comm(R, 1);
          }
@@ -202,7 +217,8 @@ int amr_driver_begin(amr_run *R)
    L->sums = (double *) amr_xmalloc(sizeof(double) * (size_t) (L->n_rows ? L->n_rows : 1) *
                                     P->num_vars);
    amr_dev_check(R, amr_dev_reserve_sums(R->dev, L->n_rows ? L->n_rows : 1), "reserve_sums");
-   amr_dev_set_timing(R->dev, 1);
+   /* exchange time per direction only when a report prints it */
+   amr_dev_set_timing(R->dev, (P->report_perf & 7) ? 2 : 1);
    double ms[4];
    amr_dev_read_timing(R->dev, ms);
 

@@ -359,6 +359,13 @@ int amr_dev_read_timing(amr_dev *d, double ms[4])
    return AMR_OK;
 }
 
+/* The CPU restatement has one interior per record: the fused stage is the
+ * plain stage (same cells and checksums by definition). */
+int amr_dev_stage_fused(amr_dev *d, int start, int num, int stage, int permute, int sum_row)
+{
+   return amr_dev_stage(d, start, num, stage, permute, sum_row);
+}
+
 int amr_dev_read_exchange_timing(amr_dev *d, double ms[3])
 {
    (void) d;

@@ -1,0 +1,72 @@
+"""Stages that read same-level faces in place (amr_dev_stage_fused).
+
+The drop-in driver runs every stage of a timestep but the last as a fused
+stage (by default when a stage has >= 1e8 cell updates; MINIAMR_FUSED=2 at
+every size, as here): the stencil reads same-level neighbours'
+interiors instead of ghost copies, writing to the second interior copy of
+each record.  Every checksum and every final cell (ghosts included) must
+be bit-identical to the plain stages' run, on every golden mesh (7- and
+27-point, permuted exchanges -- where the fused stage is the plain one --,
+variable groups, refinement with stale ghosts), and the golden records of
+the reference must still hold.  A context also refuses the calls that need
+written ghost planes until a plain exchange has run.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import amrcheck as A
+import meshgen
+import miniamr_amd as M
+from conftest import require
+
+CASES = sorted(f[:-5] for f in os.listdir(A.GOLDEN)
+               if f.endswith(".json") and not f.startswith("report_"))
+# "2": fused at every size (by default only stages of >= 1e8 cell updates
+# are); "0": the plain stages
+FUSED = {"MINIAMR_FUSED": "2"}
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_fused_stages_are_identical(name):
+    require(M.EXE_PATH, "run __graft_entry__.build()")
+    g = A.golden(name)
+    print("case: fused %s: %s" % (name, g["args"]), flush=True)
+    kind = "cells" if "cells" in g else ("bsums" if "bsums" in g else None)
+    if kind == "bsums" and "vars" in g["bsums"]:
+        kind = "bsums_only"
+    a = A.run(M.EXE_PATH, g["args"], dump=kind, timeout=110, env={"MINIAMR_FUSED": "0"})
+    b = A.run(M.EXE_PATH, g["args"], dump=kind, timeout=110, env=FUSED)
+    assert a["rc"] == b["rc"]
+    assert a["sums"] == b["sums"]
+    assert a["counts"] == b["counts"]
+    for k in ("cells", "bsums"):
+        if k in a:
+            for key in a[k]:
+                assert np.array_equal(a[k][key], b[k][key]), (k, key)
+    check_cells = not any("difference too large" in m for m in g["messages"])
+    A.compare(b, g, sum_rtol=1e-12, max_ulp=4, check_cells=check_cells)
+
+
+def test_stale_ghosts_are_refused(monkeypatch):
+    require(M.LIB_PATH, "run __graft_entry__.build()")
+    monkeypatch.setenv("MINIAMR_FUSED", "2")
+    rng = np.random.default_rng(5)
+    d = M.Device(nx=6, ny=4, nz=8, num_vars=3, max_blocks=14, stencil=27)
+    for s in range(11):
+        d.upload(s, rng.random(d.shape))
+    plan = meshgen.random_plan(rng, 5, 6)
+    order = list(range(11))
+    d.set_mesh(order, plan)
+    d.stage_fused(0, 3, stage=0)
+    with pytest.raises(M.AmrError):
+        d.set_mesh(order, plan)
+    with pytest.raises(M.AmrError):
+        d.download(0)
+    d.comm(0, 3, stage=1)          # a plain exchange writes every ghost plane again
+    d.set_mesh(order, plan)
+    d.download(0)
+    d.close()
