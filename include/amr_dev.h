@@ -63,7 +63,13 @@ enum {
    AMR_PREC_BF16 = 2     /* BF16 storage, FP32 sums, FP64 checksums */
 };
 
-/* Face-job kinds of one exchange direction (code 0 semantics). */
+/* Face-job kinds of one exchange direction.  Kinds 0-3 are the --code 0
+ * exchange (ghost cells of the face interior; the 27-point stencil extends
+ * same-level y/z faces and boundary faces over the ghost ring).  Kinds 4-7
+ * (kind | AMR_FACE_GHOSTS) are the --code 1/2 exchange, which moves whole
+ * (n+2) x (n+2) ghost planes, ghost ring included, at any stencil
+ * (ref/comm.c:1535-1579, 1689-1907, 1941-1964); on one rank codes 1 and 2
+ * differ only in their message formats, so both use these kinds. */
 enum {
    AMR_FACE_SAME = 0,     /* dst ghost plane <- src boundary plane
                              (ref/comm.c:1473-1534) */
@@ -71,7 +77,18 @@ enum {
                              (ref/comm.c:1618-1625 and the y/z twins) */
    AMR_FACE_RESTRICT = 2, /* dst is coarse: ghost quarter <- sum of 4 fine
                              cells (ref/comm.c:1626-1630 and twins) */
-   AMR_FACE_BC = 3        /* reflective boundary (ref/comm.c:1911-1965) */
+   AMR_FACE_BC = 3,       /* reflective boundary (ref/comm.c:1911-1965) */
+   AMR_FACE_GHOSTS = 4,   /* flag: whole ghost planes (--code 1/2) */
+   AMR_FACE_SAME_GHOSTS = 4,     /* (n+2)^2 plane copy (ref/comm.c:1535-1579) */
+   AMR_FACE_PROLONG_GHOSTS = 5,  /* fine ghost plane, ring included: fine
+                                    (a, b) <- coarse (a0 + (a+1)/2,
+                                    b0 + (b+1)/2) / 4 (ref/comm.c:1690-1761,
+                                    y and z at :1762-1906) */
+   AMR_FACE_RESTRICT_GHOSTS = 6, /* the coarse quarter and its share of the
+                                    ring: interior cells <- 4 fine cells,
+                                    ring edges <- 2 fine ring cells, ring
+                                    corner <- 1 (same lines) */
+   AMR_FACE_BC_GHOSTS = 7        /* reflective boundary over the whole plane */
 };
 
 typedef struct {

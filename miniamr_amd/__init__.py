@@ -27,6 +27,7 @@ INCLUDE = os.path.join(REPO, "include", "amr_dev.h")
 AMR_OK = 0
 PREC_F64, PREC_F32, PREC_BF16 = 0, 1, 2
 FACE_SAME, FACE_PROLONG, FACE_RESTRICT, FACE_BC = 0, 1, 2, 3
+FACE_GHOSTS = 4   # or'ed into a kind: whole ghost planes (--code 1/2)
 
 # Every symbol include/amr_dev.h declares.
 ABI_SYMBOLS = [

@@ -762,6 +762,8 @@ int amr_dev_tape(amr_dev *d, int on)
    if (!d)
       return AMR_ERR_ARG;
    if (on) {
+      if (d->whole_ghosts)
+         return amr_set_error(d, AMR_ERR_ARG, "tape: the adjoint covers the --code 0 exchange only");
       delete d->tape;
       d->tape = new Tape();
       d->tape->on = true;

@@ -109,6 +109,7 @@ struct amr_dev {
    // I and ZI parts, so a stage can read one and write the other.  par[v]
    // is the copy that holds variable v's current interior; oI_par / oZI_par
    // are the two copies' offsets in the record.
+   bool whole_ghosts = false;       // the plan has --code 1/2 jobs (kind & AMR_FACE_GHOSTS)
    bool two_interiors = false;
    int fused_mode = 1;              // MINIAMR_FUSED: 0 never, 1 large stages, 2 always
    int oI_par[2] = {0, 0}, oZI_par[2] = {0, 0};

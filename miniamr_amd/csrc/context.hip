@@ -344,7 +344,7 @@ int amr_set_mesh_impl(amr_dev *d, const int32_t *sorted, int n_active,
    for (int t = 0; t < nj; t++) {
       const amr_face_job &j = jobs[t];
       int dir = t < count[0] ? 0 : t < count[0] + count[1] ? 1 : 2;
-      if (j.kind < AMR_FACE_SAME || j.kind > AMR_FACE_BC || j.face / 2 != dir ||
+      if (j.kind < AMR_FACE_SAME || j.kind > AMR_FACE_BC_GHOSTS || j.face / 2 != dir ||
           j.face < 0 || j.face > 5 || j.dst < 0 || j.dst >= d->c.max_blocks ||
           j.src < 0 || j.src >= d->c.max_blocks || (j.iq & ~1) || (j.jq & ~1))
          return amr_set_error(d, AMR_ERR_ARG, "set_mesh: bad face job %d", t);
@@ -365,6 +365,9 @@ int amr_set_mesh_impl(amr_dev *d, const int32_t *sorted, int n_active,
    d->n_active = n_active;
    for (int i = 0; i < 3; i++)
       d->count[i] = count[i];
+   d->whole_ghosts = false;
+   for (int t = 0; t < nj; t++)
+      d->whole_ghosts |= (jobs[t].kind & AMR_FACE_GHOSTS) != 0;
    d->plan_version++;
    if (d->two_interiors)
       return build_fused_plan(d, sorted, n_active, jobs, count);
@@ -421,6 +424,9 @@ int amr_dev_set_mesh(amr_dev *d, const int32_t *sorted, int n_active,
    int rc = amr_set_mesh_impl(d, sorted, n_active, jobs, count);
    if (rc || !d->tape || !d->tape->on || d->replaying)
       return rc;
+   if (d->whole_ghosts)
+      return amr_set_error(d, AMR_ERR_ARG,
+                           "set_mesh: the tape covers the --code 0 exchange only");
    TapePlan pl;
    pl.sorted.assign(sorted, sorted + n_active);
    pl.jobs.assign(jobs, jobs + (count[0] + count[1] + count[2]));
@@ -650,15 +656,16 @@ int amr_dev_stage_fused(amr_dev *d, int start, int num, int stage, int permute, 
 {
    if (!d)
       return AMR_ERR_ARG;
-   // the plain stage where in-place reads cannot stand in for it, and for
+   // the plain stage where in-place reads cannot stand in for it (permuted
+   // phases, --code 1/2 planes, taping and replay), and for
    // stages below ~1e8 cell updates: those are bound by launch latency,
    // where the fused kernels' longer set-up costs more than the ghost planes
    // they save (FP32 runs of 8^3-16^3 blocks with 8-40 variables measured
    // 1-7% slower fused; the 16^3, 40-variable bandwidth-scale run 15-18%
    // faster)
    const double cells = (double) d->n_active * num * d->g.nx * d->g.ny * d->g.nz;
-   if (!d->two_interiors || permute || (d->tape && d->tape->on) || d->replaying ||
-       (d->fused_mode == 1 && cells < 1.0e8))
+   if (!d->two_interiors || permute || d->whole_ghosts || (d->tape && d->tape->on) ||
+       d->replaying || (d->fused_mode == 1 && cells < 1.0e8))
       return amr_dev_stage(d, start, num, stage, permute, sum_row);
    if (start < 0 || num < 0 || start + num > d->c.num_vars)
       return amr_set_error(d, AMR_ERR_ARG, "stage: variables out of range");

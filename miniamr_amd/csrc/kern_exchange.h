@@ -53,20 +53,27 @@ __device__ __forceinline__ void exchange_job(T *__restrict__ pool, const amr_fac
    const gptr<const T> src =
       global_view((const T *) pool) + (int64_t) v0 * L.var_stride + (int64_t) jb.src * L.rec;
    const int64_t vs = L.var_stride;
+   const int kind = jb.kind & 3;                 // AMR_FACE_SAME .. AMR_FACE_BC
+   const bool whole = jb.kind & AMR_FACE_GHOSTS;  // --code 1/2: ghost ring included
 
-   if (jb.kind != AMR_FACE_RESTRICT) {
+   if (kind != AMR_FACE_RESTRICT) {
       // SAME / BC / PROLONG: one source cell per ghost cell
       int ext_a = 0, ext_b = 0;
-      if (L.stencil == 27 && jb.kind != AMR_FACE_PROLONG) {
-         if (jb.kind == AMR_FACE_BC) ext_a = ext_b = 1;
+      if (whole) {
+         ext_a = ext_b = 1;
+      } else if (L.stencil == 27 && kind != AMR_FACE_PROLONG) {
+         if (kind == AMR_FACE_BC) ext_a = ext_b = 1;
          else if (dir == 1) ext_a = 1;
          else if (dir == 2) ext_a = ext_b = 1;
       }
       const int a0 = 1 - ext_a, b0 = 1 - ext_b;
       const int wa = na + 2 * ext_a, wb = nb + 2 * ext_b, ncell = wa * wb;
       // BC reads the block's own plane next to the face
-      const int sp = jb.kind == AMR_FACE_BC ? (hi ? n : 1) : plane;
-      const bool prolong = jb.kind == AMR_FACE_PROLONG;
+      const int sp = kind == AMR_FACE_BC ? (hi ? n : 1) : plane;
+      // prolongation over the whole plane: fine (a, b) <- coarse
+      // (ao + (a+1)/2, bo + (b+1)/2), which also maps the fine ring rows
+      // 0 and n+1 to coarse rows ao and ao + n/2 + 1
+      const bool prolong = kind == AMR_FACE_PROLONG;
       // prolongation: fine cell (a, b) of the whole face <- coarse cell of the quarter
       const int ao = jb.jq * (na / 2), bo = jb.iq * (nb / 2);
       const int G = var_groups<KV>(ncell, nthr), cw = nthr / G;
@@ -111,21 +118,35 @@ __device__ __forceinline__ void exchange_job(T *__restrict__ pool, const amr_fac
    } else {  // RESTRICT: coarse quarter cell <- sum of 4 fine cells
       // RV of the thread's variables per pass (4 loads each)
       constexpr int RV = CP * KV / 4 < 1 ? 1 : CP * KV / 4;
-      const int ha = na / 2, hb = nb / 2, ncell = ha * hb;
+      const int ha = na / 2, hb = nb / 2;
       const int ao = jb.jq * ha, bo = jb.iq * hb;
+      // whole planes: the quarter's rows/columns grow by the ring row or
+      // column on its side (coarse 0 for the low quarter, n+1 for the high
+      // one); a ring row takes the fine ring row, one cell per coarse cell
+      const int wa = ha + whole, wb = hb + whole, ncell = wa * wb;
+      const int a_lo = whole ? (jb.jq ? ha + 1 : 0) : 1 + ao;
+      const int b_lo = whole ? (jb.iq ? hb + 1 : 0) : 1 + bo;
       const int G = var_groups<KV>(ncell, nthr), cw = nthr / G;
       const int g = __builtin_amdgcn_readfirstlane(tid / cw), c0 = tid - g * cw;
       for (int t = c0; t < ncell; t += cw) {
-         const int a = 1 + t / hb, b = 1 + t % hb;
+         const int a = a_lo + t / wb, b = b_lo + t % wb;   // coarse ghost cell
+         // fine rows (and columns) summed: 2a'-1, 2a' (a' = a - ao) for an
+         // interior coarse row, the fine ring row itself for a ring row
+         const bool ra = a == 0 || a == na + 1, rb = b == 0 || b == nb + 1;
+         const int fa = ra ? a : 2 * (a - ao) - 1, fb = rb ? b : 2 * (b - bo) - 1;
+         const int nfb = rb ? 1 : 2, cnt = (ra ? 1 : 2) * nfb;
          int so[4];
 #pragma unroll
          for (int r = 0; r < 4; r++) {
+            // term r of the sum in the reference's order (row-major over
+            // the fine rows, then columns); terms past cnt repeat term 0
+            const int rr = r < cnt ? r : 0;
             int si, sj, sk;
-            canon(dir, plane, 2 * a - 1 + (r >> 1), 2 * b - 1 + (r & 1), si, sj, sk);
+            canon(dir, plane, fa + rr / nfb, fb + rr % nfb, si, sj, sk);
             so[r] = dir == 2 ? zcopy(L, si, sj, sk) : home(L, si, sj, sk);
          }
          int i, j, k;
-         canon(dir, ghost, a + ao, b + bo, i, j, k);
+         canon(dir, ghost, a, b, i, j, k);
          const int dh = home(L, i, j, k);
          const int dc = (k == 1 || k == L.nz) ? zcopy(L, i, j, k) : -1;
          // this thread's variables are g, g + G, ...
@@ -144,9 +165,11 @@ __device__ __forceinline__ void exchange_job(T *__restrict__ pool, const amr_fac
                if (q0 + m * G < nv) {
                   const gptr<T> dq = uniform_base(dst + (int64_t) (q0 + m * G) * vs);
                   A s = to_acc(v[m][0]);
-                  s = s + to_acc(v[m][1]);
-                  s = s + to_acc(v[m][2]);
-                  s = s + to_acc(v[m][3]);
+                  if (cnt > 1) s = s + to_acc(v[m][1]);
+                  if (cnt > 2) {
+                     s = s + to_acc(v[m][2]);
+                     s = s + to_acc(v[m][3]);
+                  }
                   const T o = from_acc<T>(s);
                   st(dq, dh, o);
                   if (dc >= 0) st(dq, dc, o);
@@ -171,7 +194,8 @@ __device__ __forceinline__ void exchange_job_fused(T *__restrict__ pool, const F
                                                    int nthr)
 {
    const amr_face_job &jb = fj.jb;
-   if (jb.kind == AMR_FACE_PROLONG || jb.kind == AMR_FACE_RESTRICT) {
+   if (jb.kind == AMR_FACE_PROLONG || jb.kind == AMR_FACE_RESTRICT ||
+       (jb.kind & AMR_FACE_GHOSTS)) {
       exchange_job<T, KV>(pool, jb, L, v0, nv, tid, nthr);
       return;
    }

@@ -280,12 +280,14 @@ double amr_rand01(amr_run *R)
 static int opposite(int f) { return f ^ 1; }
 
 /* Ghost planes each active block receives, per direction
- * (ref/comm.c:159-203 with on_proc_comm, on_proc_comm_diff, apply_bc). */
+ * (ref/comm.c:159-203 with on_proc_comm, on_proc_comm_diff, apply_bc).
+ * --code 1/2 jobs move whole ghost planes (AMR_FACE_GHOSTS). */
 int amr_build_exchange_plan(amr_run *R)
 {
    amr_vec *v = &R->face_jobs;
    int32_t count[3] = { 0, 0, 0 };
    int na = amr_n_active(R);
+   const int whole = R->P.code ? AMR_FACE_GHOSTS : 0;
    v->n = 0;
    for (int dir = 0; dir < 3; dir++) {
       int before = v->n;
@@ -298,14 +300,14 @@ int amr_build_exchange_plan(amr_run *R)
             j.face = f;
             j.dst = s;
             if (b->nei_level[f] == b->level) {
-               j.kind = AMR_FACE_SAME;
+               j.kind = AMR_FACE_SAME | whole;
                j.src = b->nei[f][0][0];
                amr_vec_push(v, &j);
             } else if (b->nei_level[f] == b->level + 1) {
                /* coarse side: one restriction per finer neighbour */
                for (int iq = 0; iq < 2; iq++)
                   for (int jq = 0; jq < 2; jq++) {
-                     j.kind = AMR_FACE_RESTRICT;
+                     j.kind = AMR_FACE_RESTRICT | whole;
                      j.src = b->nei[f][iq][jq];
                      j.iq = iq;
                      j.jq = jq;
@@ -317,7 +319,7 @@ int amr_build_exchange_plan(amr_run *R)
                for (int iq = 0; iq < 2 && !found; iq++)
                   for (int jq = 0; jq < 2 && !found; jq++)
                      if (R->blocks[m].nei[g][iq][jq] == s) {
-                        j.kind = AMR_FACE_PROLONG;
+                        j.kind = AMR_FACE_PROLONG | whole;
                         j.src = m;
                         j.iq = iq;
                         j.jq = jq;
@@ -325,7 +327,7 @@ int amr_build_exchange_plan(amr_run *R)
                         found = 1;
                      }
             } else if (b->nei_level[f] == AMR_BOUNDARY) {
-               j.kind = AMR_FACE_BC;
+               j.kind = AMR_FACE_BC | whole;
                j.src = s;
                amr_vec_push(v, &j);
             } else {
@@ -342,7 +344,8 @@ int amr_build_exchange_plan(amr_run *R)
    for (int t = 0, dir = 0; dir < 3; dir++)
       for (int e = 0; e < count[dir]; e++, t++) {
          const amr_face_job *jj = (const amr_face_job *) v->v + t;
-         int k = jj->kind == AMR_FACE_SAME ? 0 : jj->kind == AMR_FACE_BC ? 2 : 1;
+         int kind = jj->kind & 3;
+         int k = kind == AMR_FACE_SAME ? 0 : kind == AMR_FACE_BC ? 2 : 1;
          R->plan_faces[dir][k]++;
       }
    return amr_dev_check(R, amr_dev_set_mesh(R->dev, R->sorted_slot, na,

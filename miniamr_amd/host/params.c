@@ -6,7 +6,7 @@
  * ref/main.c:657-723 check_input), so existing miniAMR command lines run
  * unchanged.  The error messages are kept word for word because scripts
  * around miniAMR match on them.  Flags that select features outside this
- * build's scope (--stencil 0, --code 1/2, --hilbert/--trunc_hilbert
+ * build's scope (--stencil 0, --hilbert/--trunc_hilbert
  * initial orderings, plotting) are accepted by the parser and rejected by
  * amr_check_input with an explicit message rather than silently ignored.
  */
@@ -65,7 +65,7 @@ void amr_print_help(void)
       "--report_diffusion - report check sums for each variable",
       "--report_perf - bitmask: 1 yaml, 2 text, 4 stdout, 8 block counts",
       "--plot_freq - frequency (timesteps) of plotting (0 for none)",
-      "--code - communication variant (0 = minimal ghosts)",
+      "--code - communication variant (0 = minimal ghosts, 1/2 = whole ghost planes)",
       "--permute - alternate directions in communication",
       "--blocking_send, --send_faces - message options (no effect with one rank)",
       "--refine_ghost - use full extent of block (including ghosts) to determine if block is refined",
@@ -290,10 +290,6 @@ int amr_check_input(amr_params *P)
    /* scope of this build */
    if (!error && P->stencil == 0) {
       printf("this build implements --stencil 7 and 27 (the variable-work stencil 0 is not built)\n");
-      error = 1;
-   }
-   if (!error && P->code != 0) {
-      printf("this build implements --code 0 (minimal ghost exchange) only\n");
       error = 1;
    }
    if (!error && P->lb_method >= 2) {

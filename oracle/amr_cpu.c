@@ -135,13 +135,17 @@ static void plane_axes(int dir, int *sa, int *sb)
 }
 
 /* Extended in-plane ranges for the 27-point stencil (ref/comm.c:1496-1533,
- * 1941-1964): y faces extend x, z faces extend x and y, BC extends both. */
+ * 1941-1964): y faces extend x, z faces extend x and y, BC extends both.
+ * --code 1/2 jobs (AMR_FACE_GHOSTS) cover the whole (n+2)^2 plane at any
+ * stencil (ref/comm.c:1535-1579, 1941-1964). */
 static void ranges(amr_dev *d, const amr_face_job *j, int *a0, int *a1, int *b0, int *b1)
 {
    int dir = j->face / 2, sa, sb;
    plane_axes(dir, &sa, &sb);
    int ext_a = 0, ext_b = 0;
-   if (d->c.stencil == 27) {
+   if (j->kind & AMR_FACE_GHOSTS) {
+      ext_a = ext_b = 1;
+   } else if (d->c.stencil == 27) {
       if (j->kind == AMR_FACE_BC) ext_a = ext_b = 1;
       else if (dir == 1) ext_a = 1;
       else if (dir == 2) ext_a = ext_b = 1;
@@ -159,7 +163,8 @@ static void run_job(amr_dev *d, const amr_face_job *j, int v)
    plane_axes(dir, &sa, &sb);
    int ha = nsz(d, sa) / 2, hb = nsz(d, sb) / 2;
    int a0, a1, b0, b1;
-   switch (j->kind) {
+   int whole = j->kind & AMR_FACE_GHOSTS;
+   switch (j->kind & 3) {
    case AMR_FACE_SAME:
       ranges(d, j, &a0, &a1, &b0, &b1);
       for (int a = a0; a <= a1; a++)
@@ -175,6 +180,20 @@ static void run_job(amr_dev *d, const amr_face_job *j, int v)
    case AMR_FACE_PROLONG: {
       /* fine ghost quad <- coarse cell / 4; a offset jq*ha, b offset iq*hb */
       int ao = j->jq * ha, bo = j->iq * hb;
+      if (whole) {
+         /* the whole fine plane (ref/comm.c:1698-1758 for x): fine ring row
+          * 0 takes coarse row ao, fine ring row n+1 coarse row ao + ha + 1,
+          * fine rows 2a-1 and 2a coarse row ao + a; columns alike */
+         for (int a = 0; a <= 2 * ha + 1; a++) {
+            int ca = a == 0 ? ao : a == 2 * ha + 1 ? ao + ha + 1 : ao + (a + 1) / 2;
+            for (int b = 0; b <= 2 * hb + 1; b++) {
+               int cb = b == 0 ? bo : b == 2 * hb + 1 ? bo + hb + 1 : bo + (b + 1) / 2;
+               *cell(d, v, j->dst, dir, ghost, a, b) =
+                  *cell(d, v, j->src, dir, src_pl, ca, cb) / 4.0;
+            }
+         }
+         break;
+      }
       for (int a = 1; a <= ha; a++)
          for (int b = 1; b <= hb; b++) {
             double q = *cell(d, v, j->src, dir, src_pl, a + ao, b + bo) / 4.0;
@@ -187,6 +206,35 @@ static void run_job(amr_dev *d, const amr_face_job *j, int v)
    }
    case AMR_FACE_RESTRICT: {
       int ao = j->jq * ha, bo = j->iq * hb;
+      if (whole) {
+         /* the quarter plus the ring row on its side (coarse row 0 for
+          * jq 0, n+1 for jq 1) and the ring column on its side
+          * (ref/comm.c:1708-1760 for x): ring cells sum the fine ring
+          * row's (or column's) two cells, the ring corner copies one */
+         int alo = j->jq ? ha + 1 : 0, ahi = j->jq ? 2 * ha + 1 : ha;
+         int blo = j->iq ? hb + 1 : 0, bhi = j->iq ? 2 * hb + 1 : hb;
+         for (int a = alo; a <= ahi; a++)
+            for (int b = blo; b <= bhi; b++) {
+               int ra = a == 0 || a == 2 * ha + 1, rb = b == 0 || b == 2 * hb + 1;
+               int fa = 2 * (a - ao), fb = 2 * (b - bo);
+               double val;
+               if (ra && rb)
+                  val = *cell(d, v, j->src, dir, src_pl, a, b);
+               else if (ra)
+                  val = *cell(d, v, j->src, dir, src_pl, a, fb - 1) +
+                        *cell(d, v, j->src, dir, src_pl, a, fb);
+               else if (rb)
+                  val = *cell(d, v, j->src, dir, src_pl, fa - 1, b) +
+                        *cell(d, v, j->src, dir, src_pl, fa, b);
+               else
+                  val = *cell(d, v, j->src, dir, src_pl, fa - 1, fb - 1) +
+                        *cell(d, v, j->src, dir, src_pl, fa - 1, fb) +
+                        *cell(d, v, j->src, dir, src_pl, fa, fb - 1) +
+                        *cell(d, v, j->src, dir, src_pl, fa, fb);
+               *cell(d, v, j->dst, dir, ghost, a, b) = val;
+            }
+         break;
+      }
       for (int a = 1; a <= ha; a++)
          for (int b = 1; b <= hb; b++)
             *cell(d, v, j->dst, dir, ghost, a + ao, b + bo) =

@@ -72,6 +72,18 @@ CASES = {
                 "--num_refine 3 --max_blocks 3000 --time 6.0 --stages_per_ts 1 --checksum_freq 1 "
                 "--refine_freq 1 --num_objects 1 "
                 "--object 2 1 0.5 0.5 0.5 0.02 0.03 -0.01 0.25 0.25 0.25 0.002 0 0", "cells"),
+    # --code 1: whole ghost planes with the 7-point stencil, two levels, a
+    # moving sphere (prolongation/restriction of the ghost ring).
+    "c1_t7": ("--init_x 2 --init_y 1 --init_z 1 --nx 4 --ny 4 --nz 4 --num_vars 2 "
+              "--code 1 --num_refine 2 --num_tsteps 3 --stages_per_ts 3 --checksum_freq 1 "
+              "--refine_freq 1 " + SPHERE0, "cells"),
+    # --code 2, 27-point, non-cubic blocks, permuted directions, variable
+    # groups, two levels of refinement, bouncing solid sphere.
+    "c2_t27": ("--init_x 2 --init_y 2 --init_z 1 --nx 6 --ny 4 --nz 8 --num_vars 3 "
+               "--comm_vars 2 --stencil 27 --permute --code 2 --error_tol -10 --num_refine 2 "
+               "--max_blocks 3000 --num_tsteps 4 --stages_per_ts 3 --checksum_freq 1 "
+               "--refine_freq 1 --num_objects 1 "
+               "--object 3 1 0.2 0.6 0.4 0.1 -0.08 0.06 0.15 0.2 0.1 0.01 0 0", "cells"),
     # Differentiation-form mesh (static sphere, 40 stages, 4 vars).
     "cfg4": ("--init_x 2 --init_y 2 --init_z 2 --nx 8 --ny 8 --nz 8 --num_vars 4 "
              "--stencil 7 --num_refine 2 --num_tsteps 2 --stages_per_ts 20 "

@@ -10,12 +10,14 @@ that applies the jobs correctly -- in any order -- gives one answer.
 import numpy as np
 
 from miniamr_amd import FACE_JOB, SPLIT_JOB, MERGE_JOB, FACE_SAME, FACE_PROLONG, \
-    FACE_RESTRICT, FACE_BC
+    FACE_RESTRICT, FACE_BC, FACE_GHOSTS
 
 
-def random_plan(rng, n_src, n_dst, kinds=(FACE_SAME, FACE_PROLONG, FACE_RESTRICT, FACE_BC)):
+def random_plan(rng, n_src, n_dst, kinds=(FACE_SAME, FACE_PROLONG, FACE_RESTRICT, FACE_BC),
+                ghosts=False):
     """Jobs for 3 directions; sources are slots [0, n_src), destinations
-    [n_src, n_src + n_dst)."""
+    [n_src, n_src + n_dst).  ghosts: every job moves whole ghost planes
+    (the --code 1/2 kinds)."""
     by_dir = []
     for d in range(3):
         jobs = []
@@ -31,6 +33,8 @@ def random_plan(rng, n_src, n_dst, kinds=(FACE_SAME, FACE_PROLONG, FACE_RESTRICT
                 else:
                     jobs.append((kind, face, dst, int(rng.integers(0, n_src)),
                                  int(rng.integers(0, 2)), int(rng.integers(0, 2))))
+        if ghosts:
+            jobs = [(j[0] | FACE_GHOSTS,) + tuple(j[1:]) for j in jobs]
         rng.shuffle(jobs)
         by_dir.append(np.array(jobs, dtype=FACE_JOB))
     return by_dir

@@ -28,6 +28,7 @@ import pytest
 import torch
 
 import amrcheck as A
+import meshgen
 import miniamr_amd as M
 import tapegen
 from conftest import require
@@ -232,3 +233,20 @@ def test_fp32_adjoint_tracks_fp64(taped):
         assert err <= 1e-4
     finally:
         tr32.close()
+
+
+def test_tape_refuses_whole_ghost_plans():
+    """The adjoint covers the --code 0 exchange: taping a --code 1/2 plan
+    (whole ghost planes) is refused at tape start and at set_mesh."""
+    lib = _lib()
+    rng = np.random.default_rng(3)
+    kw = dict(nx=4, ny=4, nz=4, num_vars=1, max_blocks=8, stencil=7)
+    dev = M.Device(lib=lib, **kw)
+    order = list(range(6))
+    dev.set_mesh(order, meshgen.random_plan(rng, 2, 4, ghosts=True))
+    assert lib.amr_dev_tape(dev.h, 1) == -1   # AMR_ERR_ARG
+    dev2 = M.Device(lib=lib, **kw)
+    assert lib.amr_dev_tape(dev2.h, 1) == 0
+    with pytest.raises(M.AmrError):
+        dev2.set_mesh(order, meshgen.random_plan(rng, 2, 4, ghosts=True))
+    lib.amr_dev_tape(dev2.h, 0)

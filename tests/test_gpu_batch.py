@@ -42,6 +42,7 @@ RUNS = [
 ]
 COMMON27 = " --stencil 27 --error_tol -10 --stages_per_ts 6 --checksum_freq 2"
 COMMON7 = " --stencil 7 --permute --stages_per_ts 6 --checksum_freq 3"
+CODE1 = " --stencil 7 --code 1 --stages_per_ts 6 --checksum_freq 3"
 
 
 @pytest.fixture(scope="module")
@@ -58,7 +59,8 @@ SHAPES = {"wave": "1,0,0,0,8,64", "wg_all_persistent": "2,512,256,7,4,256",
           "mixed": "1,1024,0,5,16,128", "tuned": None}
 
 
-@pytest.mark.parametrize("common", [COMMON27, COMMON7], ids=["27pt", "7pt_permute"])
+@pytest.mark.parametrize("common", [COMMON27, COMMON7, CODE1],
+                         ids=["27pt", "7pt_permute", "7pt_code1"])
 @pytest.mark.parametrize("shape", sorted(SHAPES))
 def test_batch_f64_matches_oracle(exes, common, shape):
     print("case: batch f64 %s shape=%s" % (common.split()[1], shape), flush=True)

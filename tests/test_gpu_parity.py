@@ -46,15 +46,16 @@ def assert_same_cells(g, c, slots):
 
 # Block shapes: small faces (variables split over wavefront groups), the
 # 16^3 workload's, and uneven ones; 11 variables = one full and one partial
-# group of 8.
+# group of 8.  code 1: every job moves whole ghost planes (--code 1/2).
+@pytest.mark.parametrize("code", [0, 1])
 @pytest.mark.parametrize("st", [7, 27])
 @pytest.mark.parametrize("permute", [False, True])
 @pytest.mark.parametrize("n,nv,vars_", [((6, 4, 8), 3, 2), ((16, 16, 16), 11, 11),
                                         ((12, 10, 14), 11, 11)])
-def test_exchange_matches_oracle(libs, st, permute, n, nv, vars_):
-    print("case: exchange stencil=%d permute=%s block=%s vars=%d" % (st, permute, n, vars_),
-          flush=True)
-    rng = np.random.default_rng(10 + st + permute)
+def test_exchange_matches_oracle(libs, st, permute, n, nv, vars_, code):
+    print("case: exchange stencil=%d permute=%s block=%s vars=%d code=%d"
+          % (st, permute, n, vars_, code), flush=True)
+    rng = np.random.default_rng(10 + st + permute + 100 * code)
     n_src, n_dst = 5, 6
     kw = dict(nx=n[0], ny=n[1], nz=n[2], num_vars=nv, max_blocks=n_src + n_dst + 2, stencil=st)
     g, c = pair(libs, **kw)
@@ -62,7 +63,7 @@ def test_exchange_matches_oracle(libs, st, permute, n, nv, vars_):
         cells = rng.random(g.shape)
         g.upload(s, cells)
         c.upload(s, cells)
-    plan = meshgen.random_plan(rng, n_src, n_dst)
+    plan = meshgen.random_plan(rng, n_src, n_dst, ghosts=bool(code))
     order = list(range(n_src + n_dst))
     for d in (g, c):
         d.set_mesh(order, plan)
