@@ -95,7 +95,7 @@ typedef struct {
    int nx, ny, nz;     /* interior block size, each even and > 0 */
    int num_vars;       /* > 0 */
    int max_blocks;     /* number of block slots (ref: --max_blocks) */
-   int stencil;        /* 7 or 27 */
+   int stencil;        /* 7 or 27, or 0: the variable-work stencils */
    int precision;      /* AMR_PREC_* */
    int device;         /* HIP device ordinal */
 } amr_dev_config;
@@ -142,8 +142,27 @@ int amr_dev_set_mesh(amr_dev *dev, const int32_t *sorted_slots, int n_active,
  * (ref/comm.c:45-55). */
 int amr_dev_comm(amr_dev *dev, int start, int num_comm, int stage, int permute);
 
-/* 7- or 27-point averaging stencil of one variable over all active blocks. */
+/* 7- or 27-point averaging stencil of one variable over all active blocks.
+ * (A --stencil 0 context: stencil_driver(var, 0), see below.) */
 int amr_dev_stencil(amr_dev *dev, int var);
+
+/* --stencil 0, the variable-work stencils (ref/stencil.c:43-73, 147-981).
+ * amr_dev_set_materials installs the coefficients the reference draws at
+ * initialisation (ref/init.c:418-423): a1 and a0[n], n = num_vars / 4
+ * (the number of "materials"); a --stencil 0 context needs it before its
+ * first stencil.  amr_dev_stencil_stage is stencil_driver(var, calc_stage):
+ * variable 0 and variables >= 4*n take the 7-point average, the others one
+ * of six updates coupled to other variables, chosen by calc_stage % 6, and
+ * then the clamp of stencil_check; at 7 or 27 it equals amr_dev_stencil.
+ * amr_dev_stage passes its stage as the calculation stage and updates the
+ * variables of its range in order, as the reference's loop does.
+ * amr_dev_read_check_counts returns how many clamps divided (counts[0]:
+ * one divide and two adds each) and scaled (counts[1]: one multiply and one
+ * add) since the context was created -- the data-dependent operation counts
+ * of the report (ref/stencil.c:966-981). */
+int amr_dev_set_materials(amr_dev *dev, double a1, const double *a0, int n);
+int amr_dev_stencil_stage(amr_dev *dev, int var, int calc_stage);
+int amr_dev_read_check_counts(amr_dev *dev, int64_t counts[2]);
 
 /* Sum of one variable's interior cells over all active blocks (FP64). */
 int amr_dev_check_sum(amr_dev *dev, int var, double *sum);

@@ -36,7 +36,8 @@ ABI_SYMBOLS = [
     "amr_dev_check_sum", "amr_dev_split", "amr_dev_consolidate", "amr_dev_reserve_sums",
     "amr_dev_read_sums", "amr_dev_stage", "amr_dev_sync", "amr_dev_set_timing",
     "amr_dev_read_timing", "amr_dev_read_exchange_timing", "amr_dev_memory",
-    "amr_dev_stage_fused",
+    "amr_dev_stage_fused", "amr_dev_set_materials", "amr_dev_stencil_stage",
+    "amr_dev_read_check_counts",
 ]
 
 
@@ -108,6 +109,9 @@ def load_library(path=LIB_PATH):
         "amr_dev_read_timing": (ip, [vp, dp]),
         "amr_dev_read_exchange_timing": (ip, [vp, dp]),
         "amr_dev_memory": (ip, [vp, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ip)]),
+        "amr_dev_set_materials": (ip, [vp, ctypes.c_double, dp, ip]),
+        "amr_dev_stencil_stage": (ip, [vp, ip, ip]),
+        "amr_dev_read_check_counts": (ip, [vp, ctypes.POINTER(ctypes.c_int64)]),
         # include/amr_batch.h, device half
         "amr_batch_create": (ip, [ip, ip, ip, ctypes.POINTER(vp)]),
         "amr_batch_destroy": (ip, [vp]),
@@ -358,6 +362,22 @@ class Device:
 
     def stencil(self, var):
         self._check(self.lib.amr_dev_stencil(self.h, var), "stencil")
+
+    def stencil_stage(self, var, calc_stage):
+        """stencil_driver(var, calc_stage) (ref/stencil.c:43)."""
+        self._check(self.lib.amr_dev_stencil_stage(self.h, var, calc_stage), "stencil_stage")
+
+    def set_materials(self, a1, a0):
+        """--stencil 0 coefficients (ref/init.c:418-423): a1, a0[num_vars/4]."""
+        a = np.ascontiguousarray(a0, dtype=np.float64)
+        self._check(self.lib.amr_dev_set_materials(self.h, float(a1), _ptr(a, ctypes.c_double),
+                                                   len(a)), "set_materials")
+
+    def check_counts(self):
+        """stencil_check's clamps so far: (divides, scales)."""
+        c = (ctypes.c_int64 * 2)()
+        self._check(self.lib.amr_dev_read_check_counts(self.h, c), "read_check_counts")
+        return int(c[0]), int(c[1])
 
     def check_sum(self, var):
         out = ctypes.c_double()

@@ -764,6 +764,8 @@ int amr_dev_tape(amr_dev *d, int on)
    if (on) {
       if (d->whole_ghosts)
          return amr_set_error(d, AMR_ERR_ARG, "tape: the adjoint covers the --code 0 exchange only");
+      if (d->g.stencil == 0)
+         return amr_set_error(d, AMR_ERR_ARG, "tape: the adjoint covers the 7/27-point stencils");
       delete d->tape;
       d->tape = new Tape();
       d->tape->on = true;
@@ -852,7 +854,7 @@ int amr_dev_replay(amr_dev *d, int op0, int op1)
       case AMR_OP_SPLIT: rc = amr_split_impl(d, &tp->splits[op[1]], op[2]); break;
       case AMR_OP_MERGE: rc = amr_merge_impl(d, &tp->merges[op[1]], op[2]); break;
       case AMR_OP_COMM: rc = amr_comm_impl(d, op[1], op[2], op[3], op[4]); break;
-      case AMR_OP_STENCIL: rc = amr_stencil_impl(d, op[1]); break;
+      case AMR_OP_STENCIL: rc = amr_stencil_impl(d, op[1], 0); break;
       case AMR_OP_STAGE: rc = amr_stage_impl(d, op[1], op[2], op[3], op[4], -1); break;
       default: break;   // uploads: the caller provides the values
       }

@@ -110,6 +110,15 @@ struct amr_dev {
    // is the copy that holds variable v's current interior; oI_par / oZI_par
    // are the two copies' offsets in the record.
    bool whole_ghosts = false;       // the plan has --code 1/2 jobs (kind & AMR_FACE_GHOSTS)
+   // --stencil 0 (amr_dev_set_materials): mat = num_vars / 4, the
+   // coefficients a1 and a0[mat], the clamp counters and the work buffer of
+   // the Jacobi updates
+   int mat = 0;
+   double a1 = 0.0;
+   double *d_a0 = nullptr;
+   unsigned long long *d_check_counts = nullptr;
+   void *d_matwork = nullptr;
+   size_t matwork_bytes = 0;
    bool two_interiors = false;
    int fused_mode = 1;              // MINIAMR_FUSED: 0 never, 1 large stages, 2 always
    int oI_par[2] = {0, 0}, oZI_par[2] = {0, 0};
@@ -292,7 +301,7 @@ constexpr int kSpanExchangeX = 4;
 int amr_set_mesh_impl(amr_dev *d, const int32_t *sorted, int n_active, const amr_face_job *jobs,
                       const int32_t count[3]);
 int amr_comm_impl(amr_dev *d, int start, int num, int stage, int permute);
-int amr_stencil_impl(amr_dev *d, int var);
+int amr_stencil_impl(amr_dev *d, int var, int calc_stage);
 int amr_stage_impl(amr_dev *d, int start, int num, int stage, int permute, int sum_row);
 int amr_split_impl(amr_dev *d, const amr_split_job *jobs, int n);
 int amr_merge_impl(amr_dev *d, const amr_merge_job *jobs, int n);
@@ -309,7 +318,8 @@ Lay amr_lay_other(const amr_dev *d, const Lay &L);
 // Launchers implemented per source file; all enqueue on d->stream.
 int amr_launch_exchange_dir(amr_dev *d, int dir, int start, int num);
 int amr_launch_exchange_fused_dir(amr_dev *d, int dir, int start, int num);
-int amr_launch_stencil(amr_dev *d, int start, int num, bool want_partials);
+int amr_launch_stencil(amr_dev *d, int start, int num, bool want_partials, int calc_stage);
+int amr_launch_mat_stencil(amr_dev *d, int var, int calc_stage);
 // Stencil of a fused stage: interiors read from the current copy, written
 // to the other one (the variables' copy flips), same-level faces read in
 // place.

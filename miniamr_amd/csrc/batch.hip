@@ -627,7 +627,8 @@ int amr_batch_create(int device, int precision, int stencil, amr_batch **out)
    if (precision < AMR_PREC_F64 || precision > AMR_PREC_BF16)
       return berr(b, AMR_ERR_ARG, "unknown precision %d", precision);
    if (stencil != 7 && stencil != 27)
-      return berr(b, AMR_ERR_ARG, "stencil must be 7 or 27");
+      return berr(b, AMR_ERR_ARG,
+                  "stencil must be 7 or 27 (--stencil 0 runs on the single-instance driver)");
    int ndev = 0;
    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
       return berr(b, AMR_ERR_NODEV, "no HIP device available");

@@ -78,8 +78,8 @@ int amr_dev_create(const amr_dev_config *cfg, amr_dev **out)
       return amr_set_error(d, AMR_ERR_ARG, "block sizes must be even and >= 2");
    if (cfg->num_vars < 1 || cfg->max_blocks < 1)
       return amr_set_error(d, AMR_ERR_ARG, "num_vars and max_blocks must be positive");
-   if (cfg->stencil != 7 && cfg->stencil != 27)
-      return amr_set_error(d, AMR_ERR_ARG, "stencil must be 7 or 27");
+   if (cfg->stencil != 0 && cfg->stencil != 7 && cfg->stencil != 27)
+      return amr_set_error(d, AMR_ERR_ARG, "stencil must be 0, 7 or 27");
    if (cfg->precision < AMR_PREC_F64 || cfg->precision > AMR_PREC_BF16)
       return amr_set_error(d, AMR_ERR_ARG, "unknown precision %d", cfg->precision);
 
@@ -118,6 +118,8 @@ int amr_dev_create(const amr_dev_config *cfg, amr_dev **out)
    d->oZI_par[0] = d->oZI_par[1] = g.oZI;
    const char *fz = getenv("MINIAMR_FUSED");
    d->fused_mode = fz ? atoi(fz) : 1;
+   if (cfg->stencil == 0)
+      d->fused_mode = 0;   // the variable-work stencils read other variables' faces
    d->two_interiors = d->fused_mode != 0;
    if (d->two_interiors) {                           // second I and ZI
       d->oI_par[1] = o; o += g.nx * g.ny * g.nz;
@@ -140,6 +142,10 @@ int amr_dev_create(const amr_dev_config *cfg, amr_dev **out)
    HIP_TRY(d, hipMalloc(&d->d_stage, sizeof(double) * (size_t) cells * cfg->num_vars));
    d->h_pinned_bytes = sizeof(double) * std::max<size_t>(4096, (size_t) cfg->num_vars * 64);
    HIP_TRY(d, hipHostMalloc((void **) &d->h_pinned, d->h_pinned_bytes, hipHostMallocDefault));
+   if (cfg->stencil == 0) {
+      HIP_TRY(d, hipMalloc(&d->d_check_counts, 2 * sizeof(unsigned long long)));
+      HIP_TRY(d, hipMemsetAsync(d->d_check_counts, 0, 2 * sizeof(unsigned long long), d->stream));
+   }
    HIP_TRY(d, hipStreamSynchronize(d->stream));
    return AMR_OK;
 }
@@ -166,6 +172,9 @@ int amr_dev_destroy(amr_dev *d)
    hipFree(d->d_jobbuf);
    hipFree(d->d_fjobs);
    hipFree(d->d_redir);
+   hipFree(d->d_a0);
+   hipFree(d->d_check_counts);
+   hipFree(d->d_matwork);
    if (d->h_pinned)
       hipHostFree(d->h_pinned);
    if (d->stream && d->owns_stream)
@@ -404,14 +413,14 @@ int amr_comm_impl(amr_dev *d, int start, int num, int stage, int permute)
    return AMR_OK;
 }
 
-int amr_stencil_impl(amr_dev *d, int var)
+int amr_stencil_impl(amr_dev *d, int var, int calc_stage)
 {
-   if (!d || var < 0 || var >= d->c.num_vars)
+   if (!d || var < 0 || var >= d->c.num_vars || calc_stage < 0)
       return amr_set_error(d, AMR_ERR_ARG, "stencil: variable %d out of range", var);
    if (int rc = amr_require_fresh(d, var, 1, "stencil"))
       return rc;
    amr_span_begin(d, 1);
-   int rc = amr_launch_stencil(d, var, 1, false);
+   int rc = amr_launch_stencil(d, var, 1, false, calc_stage);
    amr_span_end(d);
    return rc;
 }
@@ -448,10 +457,49 @@ int amr_dev_comm(amr_dev *d, int start, int num, int stage, int permute)
 
 int amr_dev_stencil(amr_dev *d, int var)
 {
-   int rc = amr_stencil_impl(d, var);
+   int rc = amr_stencil_impl(d, var, 0);
    if (!rc)
       amr_tape_record(d, AMR_OP_STENCIL, var);
    return rc;
+}
+
+int amr_dev_stencil_stage(amr_dev *d, int var, int calc_stage)
+{
+   int rc = amr_stencil_impl(d, var, calc_stage);
+   if (!rc)
+      amr_tape_record(d, AMR_OP_STENCIL, var);
+   return rc;
+}
+
+int amr_dev_set_materials(amr_dev *d, double a1, const double *a0, int n)
+{
+   if (!d || d->g.stencil != 0)
+      return amr_set_error(d, AMR_ERR_ARG, "set_materials: the context is not a --stencil 0 one");
+   if (!a0 || n != d->c.num_vars / 4 || n < 1)
+      return amr_set_error(d, AMR_ERR_ARG, "set_materials: %d coefficients, need num_vars/4 = %d",
+                           n, d->c.num_vars / 4);
+   HIP_TRY(d, hipStreamSynchronize(d->stream));
+   if (!d->d_a0)
+      HIP_TRY(d, hipMalloc(&d->d_a0, sizeof(double) * n));
+   HIP_TRY(d, hipMemcpy(d->d_a0, a0, sizeof(double) * n, hipMemcpyHostToDevice));
+   d->mat = n;
+   d->a1 = a1;
+   return AMR_OK;
+}
+
+int amr_dev_read_check_counts(amr_dev *d, int64_t counts[2])
+{
+   if (!d || !counts)
+      return AMR_ERR_ARG;
+   counts[0] = counts[1] = 0;
+   if (!d->d_check_counts)
+      return AMR_OK;
+   unsigned long long h[2];
+   HIP_TRY(d, hipStreamSynchronize(d->stream));
+   HIP_TRY(d, hipMemcpy(h, d->d_check_counts, sizeof(h), hipMemcpyDeviceToHost));
+   counts[0] = (int64_t) h[0];
+   counts[1] = (int64_t) h[1];
+   return AMR_OK;
 }
 
 int amr_dev_check_sum(amr_dev *d, int var, double *sum)
@@ -631,7 +679,7 @@ int amr_stage_impl(amr_dev *d, int start, int num, int stage, int permute, int s
    if (rc)
       return rc;
    amr_span_begin(d, 1);
-   rc = amr_launch_stencil(d, start, num, sum_row >= 0);
+   rc = amr_launch_stencil(d, start, num, sum_row >= 0, stage);
    amr_span_end(d);
    if (rc || sum_row < 0)
       return rc;

@@ -61,7 +61,7 @@ __device__ __forceinline__ void exchange_job(T *__restrict__ pool, const amr_fac
       int ext_a = 0, ext_b = 0;
       if (whole) {
          ext_a = ext_b = 1;
-      } else if (L.stencil == 27 && kind != AMR_FACE_PROLONG) {
+      } else if (L.stencil != 7 && kind != AMR_FACE_PROLONG) {   // 27 and 0 extend
          if (kind == AMR_FACE_BC) ext_a = ext_b = 1;
          else if (dir == 1) ext_a = 1;
          else if (dir == 2) ext_a = ext_b = 1;
@@ -205,7 +205,7 @@ __device__ __forceinline__ void exchange_job_fused(T *__restrict__ pool, const F
    const int nb = dir == 2 ? L.ny : L.nz;
    const int ghost = hi ? n + 1 : 0;
    int ext_a = 0, ext_b = 0;
-   if (L.stencil == 27) {
+   if (L.stencil != 7) {
       if (jb.kind == AMR_FACE_BC) ext_a = ext_b = 1;
       else if (dir == 1) ext_a = 1;
       else if (dir == 2) ext_a = ext_b = 1;

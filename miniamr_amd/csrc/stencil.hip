@@ -312,8 +312,35 @@ int amr_launch_stencil_fused(amr_dev *d, int start, int num, bool want_partials)
    }
 }
 
-int amr_launch_stencil(amr_dev *d, int start, int num, bool want_partials)
+// --stencil 0 (ref/stencil.c:43-73): variables in order, since each may read
+// the variables before it after their update of this stage; variable 0 and
+// variables from 4*mat on take the 7-point average, the others the
+// variable-work updates.  Checksum partials come from a block-sum pass.
+template <typename T>
+int launch_stencil0(amr_dev *d, int start, int num, bool want_partials, int calc_stage)
 {
+   if (!d->n_active || !num)
+      return AMR_OK;
+   for (int v = start; v < start + num; v++) {
+      int rc = (v == 0 || v >= 4 * d->mat) ? launch_st_v<T, 7, 1>(d, v, 1, nullptr)
+                                           : amr_launch_mat_stencil(d, v, calc_stage);
+      if (rc)
+         return rc;
+   }
+   return want_partials ? launch_block_sums<T>(d, start, num) : AMR_OK;
+}
+
+int amr_launch_stencil(amr_dev *d, int start, int num, bool want_partials, int calc_stage)
+{
+   if (d->g.stencil == 0) {
+      if (!d->mat)
+         return amr_set_error(d, AMR_ERR_STATE, "stencil 0: amr_dev_set_materials not called");
+      switch (d->c.precision) {
+      case AMR_PREC_F64: return launch_stencil0<double>(d, start, num, want_partials, calc_stage);
+      case AMR_PREC_F32: return launch_stencil0<float>(d, start, num, want_partials, calc_stage);
+      default: return launch_stencil0<uint16_t>(d, start, num, want_partials, calc_stage);
+      }
+   }
    switch (d->c.precision) {
    case AMR_PREC_F64: return launch_stencil<double>(d, start, num, want_partials);
    case AMR_PREC_F32: return launch_stencil<float>(d, start, num, want_partials);

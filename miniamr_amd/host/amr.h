@@ -112,7 +112,10 @@ typedef struct amr_run {
    /* counters for the report */
    int64_t total_blocks, nb_min, nb_max;
    int64_t num_refined, num_reformed, total_red;
-   double total_fp_divs, total_fp_adds;
+   double total_fp_divs, total_fp_adds, total_fp_muls;   /* --stencil 0 */
+   /* --stencil 0 coefficients (ref/init.c:418-423): mat = num_vars / 4 */
+   int mat;
+   double a1, *a0;
 
    /* timers (seconds) in the reference's report slots */
    double t_all, t_init, t_refine_all, t_comm, t_calc, t_cs, t_main;

@@ -14,6 +14,9 @@
  *                          stages of the timestep, then the next group, so
  *                          a group's cells stay in the device's last-level
  *                          cache across stages.
+ * The variable-work stencils of --stencil 0 couple variables (each reads
+ * others, some already updated in the stage), so that mode always runs
+ * stage-major, variables in order within a stage.
  * The checksums of the timestep land in a device table and are then
  * checked on the host in the reference's order, printing the same lines.
  * The only observable difference is after a failed conservation check:
@@ -103,6 +106,43 @@ static void enqueue_stage(amr_run *R, int start, int number, int stage, int comm
                     "stage");
 }
 
+/* --stencil 0 operation counts of one stage's stencils of variables
+ * [start, start+number) (ref/stencil.c:100-101 for the 7-point average,
+ * :147-958 for the variable-work updates): per variable, by calculation
+ * stage % 6 and class, per cell of every active block.  The clamp's
+ * data-dependent counts (ref/stencil.c:966-981) come from the device. */
+static void count_flops0(amr_run *R, int start, int number, int calc_stage)
+{
+   const amr_params *P = &R->P;
+   const double cells = (double) R->global_active * P->nx * P->ny * P->nz;
+   const int mat = R->mat, op = calc_stage % 6;
+   for (int var = start; var < start + number; var++) {
+      double adds, muls = 0.0, divs = 1.0;
+      const int cls = var / mat;
+      if (var == 0 || var >= 4 * mat)
+         adds = 6.0;
+      else if (op == 0) {
+         if (var == 1) { adds = mat; muls = mat; divs = 0.0; }
+         else if (cls == 0) { adds = 3.0; muls = 2.0; divs = 0.0; }
+         else if (cls == 1) { adds = 3.0; muls = 3.0; }
+         else if (cls == 2) { adds = 4.0; muls = 3.0; }
+         else { adds = 6.0; muls = 6.0; }
+      } else if (op <= 3) {
+         if (var == 1) { adds = mat + 1; muls = mat; }
+         else if (cls == 0) { adds = 4.0; muls = 2.0; }
+         else { adds = 12.0; muls = 3.0; }
+      } else if (op == 4) {
+         adds = 7.0;
+         muls = 8.0;
+      } else {
+         adds = 27.0;
+      }
+      R->total_fp_adds += adds * cells;
+      R->total_fp_muls += muls * cells;
+      R->total_fp_divs += divs * cells;
+   }
+}
+
 /* Enqueue one timestep; checksum event e of the timestep goes to row e. */
 static void enqueue_timestep(amr_run *R, int comm_stage0)
 {
@@ -115,6 +155,8 @@ static void enqueue_timestep(amr_run *R, int comm_stage0)
          for (int start = 0; start < V; start += P->comm_vars) {
             int number = (start + P->comm_vars > V) ? V - start : P->comm_vars;
             enqueue_stage(R, start, number, stage, comm_stage0 + stage, row);
+            if (P->stencil == 0)
+               count_flops0(R, start, number, comm_stage0 + stage);
             R->loop.stage_calls++;
             amr_count_comm(R, 1);
          }

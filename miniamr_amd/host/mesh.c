@@ -127,6 +127,7 @@ void amr_free(amr_run *R)
 {
    if (R->dev && !R->batch)   /* batch members are destroyed with the batch */
       amr_dev_destroy(R->dev);
+   free(R->a0);
    free(R->blocks); free(R->parents); free(R->sorted_number); free(R->sorted_slot);
    free(R->sorted_index); free(R->num_blocks); free(R->block_start);
    free(R->p8); free(R->p2); free(R->grid_sum); free(R->scratch_i); free(R->cellbuf);
@@ -205,6 +206,17 @@ int amr_init(amr_run *R)
    R->init_x = P->init_block_x;
    R->init_y = P->init_block_y;
    R->init_z = P->init_block_z;
+
+   /* --stencil 0: the material coefficients, drawn from the generator
+    * before the initial values (ref/init.c:418-423) */
+   if (P->stencil == 0) {
+      R->mat = P->num_vars / 4;
+      R->a1 = amr_rand01(R);
+      R->a0 = (double *) amr_xmalloc(sizeof(double) * R->mat);
+      for (int i = 0; i < R->mat; i++)
+         R->a0[i] = amr_rand01(R);
+      amr_dev_check(R, amr_dev_set_materials(R->dev, R->a1, R->a0, R->mat), "set_materials");
+   }
 
    R->num_active = R->max_active_block = n0;
    R->global_active = n0;

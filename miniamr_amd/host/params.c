@@ -6,7 +6,7 @@
  * ref/main.c:657-723 check_input), so existing miniAMR command lines run
  * unchanged.  The error messages are kept word for word because scripts
  * around miniAMR match on them.  Flags that select features outside this
- * build's scope (--stencil 0, --hilbert/--trunc_hilbert
+ * build's scope (--hilbert/--trunc_hilbert
  * initial orderings, plotting) are accepted by the parser and rejected by
  * amr_check_input with an explicit message rather than silently ignored.
  */
@@ -217,6 +217,8 @@ int amr_parse_args(amr_params *P, int argc, char **argv)
    }
    const char *sch = getenv("MINIAMR_SCHEDULE");
    P->schedule = (sch && !strcmp(sch, "var")) ? 1 : 0;
+   if (P->stencil == 0)   /* coupled variables: keep the stage-major order */
+      P->schedule = 0;
    const char *dev = getenv("MINIAMR_DEVICE");
    P->device = dev ? atoi(dev) : 0;
    return 0;
@@ -288,10 +290,6 @@ int amr_check_input(amr_params *P)
       error = 1;
    }
    /* scope of this build */
-   if (!error && P->stencil == 0) {
-      printf("this build implements --stencil 7 and 27 (the variable-work stencil 0 is not built)\n");
-      error = 1;
-   }
    if (!error && P->lb_method >= 2) {
       printf("this build implements the RCB/Morton initial block order only (not --hilbert)\n");
       error = 1;

@@ -104,6 +104,17 @@ static void collect(const amr_run *R, amr_rep *r)
    r->fp_divs = updates;
    r->fp_adds = (P->stencil == 7 ? 6.0 : 26.0) * updates;
    r->fp_muls = 0.0;
+   if (P->stencil == 0) {
+      /* counted per stage by the driver, plus the clamps: each divide costs
+       * a divide and two adds, each scale a multiply and an add
+       * (ref/stencil.c:966-981) */
+      int64_t c[2];
+      if (amr_dev_read_check_counts(R->dev, c) != AMR_OK)
+         amr_fatal("ERROR: device read_check_counts failed: %s\n", amr_dev_last_error(R->dev));
+      r->fp_divs = R->total_fp_divs + (double) c[0];
+      r->fp_adds = R->total_fp_adds + 2.0 * (double) c[0] + (double) c[1];
+      r->fp_muls = R->total_fp_muls + (double) c[1];
+   }
    r->split = (double) R->num_refined;
    r->reformed = (double) R->num_reformed;
    r->parents = (double) R->num_parents;

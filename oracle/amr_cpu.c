@@ -10,6 +10,8 @@
  *
  * Functions and the reference code they restate:
  *   stencil_7 / stencil_27   ref/stencil.c:82-101 / 102-144
+ *   stencil 0 (stencil0_*)   ref/stencil.c:43-73, 147-981 (variable work)
+ *   fill_*, --code 1/2       ref/comm.c:1535-1579, 1689-1907
  *   fill_same                ref/comm.c:1473-1534 (on_proc_comm, code 0)
  *   fill_prolong / restrict  ref/comm.c:1597-1688 (on_proc_comm_diff, code 0)
  *   fill_bc                  ref/comm.c:1911-1965 (apply_bc)
@@ -19,6 +21,7 @@
  * Parity: pinned by the golden fixtures of tests/golden (checksums at
  * %.17g and full cell dumps of the reference on the same inputs).
  */
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -38,6 +41,9 @@ struct amr_dev {
    double *sums;
    int sum_rows;
    double *work;                /* one slab, stencil scratch */
+   int mat;                     /* --stencil 0: num_vars / 4 materials */
+   double a1, *a0;
+   int64_t checks[2];           /* stencil_check: divides, scales */
    char err[256];
 };
 
@@ -57,7 +63,8 @@ int amr_dev_create(const amr_dev_config *cfg, amr_dev **out)
    *out = NULL;
    if (!cfg || cfg->nx < 2 || cfg->ny < 2 || cfg->nz < 2 || (cfg->nx | cfg->ny | cfg->nz) & 1 ||
        cfg->num_vars < 1 || cfg->max_blocks < 1 ||
-       (cfg->stencil != 7 && cfg->stencil != 27) || cfg->precision != AMR_PREC_F64)
+       (cfg->stencil != 0 && cfg->stencil != 7 && cfg->stencil != 27) ||
+       cfg->precision != AMR_PREC_F64)
       return AMR_ERR_ARG;
    amr_dev *d = (amr_dev *) calloc(1, sizeof(amr_dev));
    d->c = *cfg;
@@ -78,6 +85,7 @@ int amr_dev_destroy(amr_dev *d)
 {
    if (!d) return AMR_OK;
    free(d->pool); free(d->work); free(d->sorted); free(d->jobs); free(d->sums);
+   free(d->a0);
    free(d);
    return AMR_OK;
 }
@@ -145,7 +153,7 @@ static void ranges(amr_dev *d, const amr_face_job *j, int *a0, int *a1, int *b0,
    int ext_a = 0, ext_b = 0;
    if (j->kind & AMR_FACE_GHOSTS) {
       ext_a = ext_b = 1;
-   } else if (d->c.stencil == 27) {
+   } else if (d->c.stencil != 7) {   /* 27, and 0 (ref/comm.c:1496, 1941) */
       if (j->kind == AMR_FACE_BC) ext_a = ext_b = 1;
       else if (dir == 1) ext_a = 1;
       else if (dir == 2) ext_a = ext_b = 1;
@@ -263,14 +271,14 @@ int amr_dev_comm(amr_dev *d, int start, int num, int stage, int permute)
    return AMR_OK;
 }
 
-int amr_dev_stencil(amr_dev *d, int var)
+static void stencil_avg(amr_dev *d, int var, int st)
 {
    int nx = d->c.nx, ny = d->c.ny, nz = d->c.nz;
    double *w = d->work;
 #define W(i, j, k) w[((size_t) (i) * d->gy + (j)) * d->gz + (k)]
    for (int in = 0; in < d->n_active; in++) {
       int s = d->sorted[in];
-      if (d->c.stencil == 7) {
+      if (st == 7) {
          for (int i = 1; i <= nx; i++)
             for (int j = 1; j <= ny; j++)
                for (int k = 1; k <= nz; k++)
@@ -305,6 +313,190 @@ int amr_dev_stencil(amr_dev *d, int var)
                A(d, var, s, i, j, k) = W(i, j, k);
    }
 #undef W
+}
+
+/* ---- --stencil 0: the variable-work stencils -------------------------
+ * Blocks in sorted order, cells in the reference's i, j, k loop order,
+ * updated in place as the reference does (the sweeps of stages 1-3 thus
+ * read the new value of the cell before), Jacobi stages 4-5 through the
+ * work slab; then stencil_check over every block. */
+
+/* class partners in units of mat: 7-point x, y, z neighbours; 27-point
+ * neighbours with 1, 2, 3 non-zero offsets (ref/stencil.c:661-958) */
+static const int kMatPartner[4][3] = { {1, 2, 3}, {1, 2, -1}, {1, -2, -1}, {-3, -2, -1} };
+
+/* stage 0, ref/stencil.c:147-225 */
+static double stencil0_point(amr_dev *d, int var, int s, int i, int j, int k)
+{
+#define P(u) A(d, (u), s, i, j, k)
+   const int mat = d->mat;
+   const double a1 = d->a1, *a0 = d->a0;
+   if (var == 1) {
+      double r = P(1);
+      for (int u = mat; u < 2 * mat; u++)
+         r += P(u) * P(0);
+      return r;
+   }
+   double v = P(var);
+   if (var < mat)
+      return v + v * (P(0) + P(1) - a1 * v);
+   if (var < 2 * mat)
+      return v * (P(0) + v + a1 * P(var + mat) + (1.0 - a1) * P(var + 2 * mat)) / P(1);
+   if (var < 3 * mat)
+      return v + P(var - mat) * (a1 * P(0) + a0[var - 2 * mat] * v + (1.0 - a1) * P(var + mat)) /
+                 P(1);
+   return v + P(var - 2 * mat) * (a1 * P(0) + a0[var - 3 * mat] * v +
+                                  (1.0 - a0[var - 3 * mat]) * P(var - mat) +
+                                  (1.0 - a1) * P(var - 2 * mat)) / (P(1) * P(1));
+}
+
+/* stages 1-3 (ref/stencil.c:233-660): axis (ex, ey, ez); cells before this
+ * one along the axis already hold their new value */
+static double stencil0_sweep(amr_dev *d, int var, int s, int i, int j, int k, int ex, int ey,
+                             int ez)
+{
+   const int mat = d->mat;
+   const double a1 = d->a1, *a0 = d->a0;
+   if (var == 1) {
+      double r = P(1);
+      for (int u = 2; u < mat + 2; u++)
+         r += P(u) * P(0);
+      return r / (a1 + r);
+   }
+   double v = P(var);
+   if (var < mat)
+      return v + v * (P(0) + P(1) - a1 * v) / (a0[var] + P(1));
+   const int cls = var / mat, g = cls == 1 ? var : var - (cls - 1) * mat;
+   double vm = A(d, var, s, i - ex, j - ey, k - ez), vp = A(d, var, s, i + ex, j + ey, k + ez);
+   double tmp1 = fabs(P(g) - A(d, g, s, i - ex, j - ey, k - ez));
+   double tmp2 = fabs(P(g) - A(d, g, s, i + ex, j + ey, k + ez));
+   double den;
+   if (cls == 1)
+      den = a1 + a0[var - mat] + vm + v + vp + P(0) + P(1);
+   else if (cls == 2)
+      den = a1 + a0[var - 2 * mat] + P(var - mat) + P(var + mat) + vm + v + vp;
+   else
+      den = a1 + a0[var - 3 * mat] + P(var - mat) + P(var - 2 * mat) + vm + v + vp;
+   if (tmp1 > tmp2)
+      return (tmp1 * vm + (tmp1 - tmp2) * (v + P(1)) + tmp2 * vp) / den;
+   return (tmp1 * vm + (tmp2 - tmp1) * (v + P(1)) + tmp2 * vp) / den;
+}
+
+/* stage 4 (ref/stencil.c:661-785) */
+static double stencil0_7(amr_dev *d, int var, int s, int i, int j, int k)
+{
+   const int c = var / d->mat, m = d->mat;
+   const int qx = var + kMatPartner[c][0] * m, qy = var + kMatPartner[c][1] * m,
+             qz = var + kMatPartner[c][2] * m;
+   return (A(d, var, s, i - 1, j, k) * A(d, qx, s, i - 1, j, k) +
+           A(d, var, s, i, j - 1, k) * A(d, qy, s, i, j - 1, k) +
+           A(d, var, s, i, j, k - 1) * A(d, qz, s, i, j, k - 1) +
+           P(var) * P(var) +
+           A(d, var, s, i, j, k + 1) * A(d, qz, s, i, j, k + 1) +
+           A(d, var, s, i, j + 1, k) * A(d, qy, s, i, j + 1, k) +
+           A(d, var, s, i + 1, j, k) * A(d, qx, s, i + 1, j, k)) / 7.0 * (d->a1 + P(var));
+}
+
+/* stage 5 (ref/stencil.c:787-958): 27 terms in i, j, k order */
+static double stencil0_27(amr_dev *d, int var, int s, int i, int j, int k)
+{
+   const int c = var / d->mat, m = d->mat;
+   double sum = 0.0;
+   int first = 1;
+   for (int di = -1; di <= 1; di++)
+      for (int dj = -1; dj <= 1; dj++)
+         for (int dk = -1; dk <= 1; dk++) {
+            int nzo = (di != 0) + (dj != 0) + (dk != 0);
+            int u = nzo ? var + kMatPartner[c][nzo - 1] * m : var;
+            double e = A(d, u, s, i + di, j + dj, k + dk);
+            sum = first ? e : sum + e;
+            first = 0;
+         }
+   return sum / (d->a1 + 27.0);
+}
+#undef P
+
+static void stencil0(amr_dev *d, int var, int calc_stage)
+{
+   int nx = d->c.nx, ny = d->c.ny, nz = d->c.nz, op = calc_stage % 6;
+   if (var == 0 || var >= 4 * d->mat) {   /* ref/stencil.c:45-47, 71-72 */
+      stencil_avg(d, var, 7);
+      return;
+   }
+   double *w = d->work;
+#define W(i, j, k) w[((size_t) (i) * d->gy + (j)) * d->gz + (k)]
+   for (int in = 0; in < d->n_active; in++) {
+      int s = d->sorted[in];
+      for (int i = 1; i <= nx; i++)
+         for (int j = 1; j <= ny; j++)
+            for (int k = 1; k <= nz; k++) {
+               if (op == 0)
+                  A(d, var, s, i, j, k) = stencil0_point(d, var, s, i, j, k);
+               else if (op <= 3)
+                  A(d, var, s, i, j, k) =
+                     stencil0_sweep(d, var, s, i, j, k, op == 1, op == 2, op == 3);
+               else
+                  W(i, j, k) = op == 4 ? stencil0_7(d, var, s, i, j, k)
+                                       : stencil0_27(d, var, s, i, j, k);
+            }
+      if (op >= 4)
+         for (int i = 1; i <= nx; i++)
+            for (int j = 1; j <= ny; j++)
+               for (int k = 1; k <= nz; k++)
+                  A(d, var, s, i, j, k) = W(i, j, k);
+   }
+#undef W
+   /* stencil_check, ref/stencil.c:966-981 */
+   for (int in = 0; in < d->n_active; in++) {
+      int s = d->sorted[in];
+      for (int i = 1; i <= nx; i++)
+         for (int j = 1; j <= ny; j++)
+            for (int k = 1; k <= nz; k++) {
+               double *c = &A(d, var, s, i, j, k);
+               *c = fabs(*c);
+               if (*c >= 1.0) {
+                  *c /= (d->a1 + d->a0[0] + *c);
+                  d->checks[0]++;
+               } else if (*c < 0.1) {
+                  *c *= 10.0 - d->a1;
+                  d->checks[1]++;
+               }
+            }
+   }
+}
+
+int amr_dev_stencil_stage(amr_dev *d, int var, int calc_stage)
+{
+   if (var < 0 || var >= d->c.num_vars || calc_stage < 0)
+      return fail(d, AMR_ERR_ARG, "stencil: variable out of range");
+   if (d->c.stencil != 0) {
+      stencil_avg(d, var, d->c.stencil);
+      return AMR_OK;
+   }
+   if (!d->a0)
+      return fail(d, AMR_ERR_STATE, "stencil 0: amr_dev_set_materials not called");
+   stencil0(d, var, calc_stage);
+   return AMR_OK;
+}
+
+int amr_dev_stencil(amr_dev *d, int var) { return amr_dev_stencil_stage(d, var, 0); }
+
+int amr_dev_set_materials(amr_dev *d, double a1, const double *a0, int n)
+{
+   if (d->c.stencil != 0 || !a0 || n < 1 || n != d->c.num_vars / 4)
+      return fail(d, AMR_ERR_ARG, "set_materials");
+   free(d->a0);
+   d->a0 = (double *) malloc(sizeof(double) * n);
+   memcpy(d->a0, a0, sizeof(double) * n);
+   d->mat = n;
+   d->a1 = a1;
+   return AMR_OK;
+}
+
+int amr_dev_read_check_counts(amr_dev *d, int64_t counts[2])
+{
+   counts[0] = d->checks[0];
+   counts[1] = d->checks[1];
    return AMR_OK;
 }
 
@@ -391,7 +583,7 @@ int amr_dev_stage(amr_dev *d, int start, int num, int stage, int permute, int su
    if (sum_row >= d->sum_rows)
       return fail(d, AMR_ERR_ARG, "sum row");
    for (int v = start; v < start + num; v++) {
-      amr_dev_stencil(d, v);
+      amr_dev_stencil_stage(d, v, stage);
       if (sum_row >= 0)
          amr_dev_check_sum(d, v, &d->sums[(size_t) sum_row * d->c.num_vars + v]);
    }
@@ -458,6 +650,10 @@ int amr_batch_create(int device, int precision, int stencil, amr_batch **out)
    b->shape.vpw = 1;
    b->shape.exchange_vars = 8;
    b->shape.exchange_threads = 128;
+   if (stencil != 7 && stencil != 27) {
+      snprintf(b->err, sizeof(b->err), "stencil must be 7 or 27");
+      return AMR_ERR_ARG;
+   }
    if (precision != AMR_PREC_F64) {
       snprintf(b->err, sizeof(b->err), "the CPU oracle is FP64 only");
       return AMR_ERR_ARG;

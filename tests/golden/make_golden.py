@@ -84,6 +84,18 @@ CASES = {
                "--max_blocks 3000 --num_tsteps 4 --stages_per_ts 3 --checksum_freq 1 "
                "--refine_freq 1 --num_objects 1 "
                "--object 3 1 0.2 0.6 0.4 0.1 -0.08 0.06 0.15 0.2 0.1 0.01 0 0", "cells"),
+    # --stencil 0: the variable-work stencils, 13 variables (mat = 3: variable
+    # 1, every class, and variable 12 on the 7-point average), 14 stages
+    # (every calculation stage % 6 twice), one level of refinement.
+    "s0_a": ("--init_x 2 --init_y 1 --init_z 1 --nx 4 --ny 4 --nz 4 --num_vars 13 "
+             "--stencil 0 --num_refine 1 --num_tsteps 2 --stages_per_ts 7 --checksum_freq 1 "
+             "--refine_freq 1 " + SPHERE0, "cells"),
+    # --stencil 0 with non-cubic blocks, variable groups, permuted directions,
+    # --code 1, one level of refinement, a bouncing solid sphere (mat = 2).
+    "s0_b": ("--init_x 1 --init_y 2 --init_z 1 --nx 6 --ny 4 --nz 8 --num_vars 9 "
+             "--comm_vars 4 --stencil 0 --permute --code 1 --num_refine 1 --num_tsteps 3 "
+             "--stages_per_ts 5 --checksum_freq 2 --refine_freq 1 --num_objects 1 "
+             "--object 3 1 0.2 0.6 0.4 0.1 -0.08 0.06 0.15 0.2 0.1 0.01 0 0", "cells"),
     # Differentiation-form mesh (static sphere, 40 stages, 4 vars).
     "cfg4": ("--init_x 2 --init_y 2 --init_z 2 --nx 8 --ny 8 --nz 8 --num_vars 4 "
              "--stencil 7 --num_refine 2 --num_tsteps 2 --stages_per_ts 20 "

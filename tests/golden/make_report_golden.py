@@ -39,6 +39,11 @@ CASES = {
     "uniform": "--init_x 2 --init_y 2 --init_z 1 --nx 4 --ny 4 --nz 4 --num_vars 2 "
                "--uniform_refine 1 --num_refine 1 --num_tsteps 3 --stages_per_ts 4 "
                "--checksum_freq 2",
+    # the variable-work stencils (--stencil 0) with --code 2: the operation
+    # counts include stencil_check's data-dependent ones
+    "stencil0": "--nx 4 --ny 4 --nz 4 --num_vars 9 --stencil 0 --code 2 --num_refine 1 "
+                "--num_tsteps 3 --stages_per_ts 6 --checksum_freq 2 --refine_freq 1 "
+                "--num_objects 1 --object 2 0 0.3 0.3 0.3 0.05 0.05 0.05 0.2 0.2 0.2 0 0 0",
     # simulated time instead of a timestep count, a hemispheroid that bounces
     "timed": "--nx 4 --ny 4 --nz 4 --num_vars 3 --num_refine 2 --time 0.5 "
              "--stages_per_ts 3 --checksum_freq 1 --refine_freq 1 --num_objects 1 "

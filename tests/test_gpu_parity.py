@@ -97,6 +97,37 @@ def test_stencil_and_checksum_match_oracle(libs, st, n):
         assert abs(a - b) <= 1e-13 * abs(b)
 
 
+# --stencil 0: every calculation stage % 6 twice on 13 variables (mat = 3:
+# variable 1, all classes, variable 12 on the 7-point average), inputs in
+# [0, 2) so that both clamp branches fire; shapes with more lines than
+# threads for the sweeps (30 x 30) and uneven axes.
+@pytest.mark.parametrize("n", [(4, 4, 4), (16, 16, 16), (6, 30, 30), (10, 6, 20)])
+def test_stencil0_matches_oracle(libs, n):
+    print("case: stencil 0 block=%s" % (n,), flush=True)
+    rng = np.random.default_rng(40)
+    nv, nb = 13, 4
+    kw = dict(nx=n[0], ny=n[1], nz=n[2], num_vars=nv, max_blocks=nb + 2, stencil=0)
+    g, c = pair(libs, **kw)
+    a0 = rng.random(nv // 4)
+    a1 = float(rng.random())
+    for d in (g, c):
+        d.set_materials(a1, a0)
+    for s in range(nb):
+        cells = 2.0 * rng.random(g.shape)
+        g.upload(s + 1, cells)
+        c.upload(s + 1, cells)
+    order = [3, 1, 4, 2]
+    for d in (g, c):
+        d.set_mesh(order, [np.zeros(0, M.FACE_JOB)] * 3)
+    for stage in range(12):
+        for d in (g, c):
+            for v in range(nv):
+                d.stencil_stage(v, stage)
+    assert_same_cells(g, c, order)
+    assert g.check_counts() == c.check_counts()
+    assert min(c.check_counts()) > 0
+
+
 def test_stage_with_checksum_table(libs):
     rng = np.random.default_rng(30)
     n_src, n_dst = 4, 4

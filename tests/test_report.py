@@ -20,7 +20,7 @@ import pytest
 from amrcheck import GOLDEN, ORACLE_X, PRODUCT_X
 import reportmask
 
-CASES = ["sphere7", "mixed27", "uniform", "timed"]
+CASES = ["sphere7", "mixed27", "uniform", "timed", "stencil0"]
 
 
 def check(exe, name):
