@@ -210,26 +210,22 @@ template <typename T> __device__ __forceinline__ void st(gptr<T> base, uint32_t 
    *(gptr<T>) ((gptr<char>) base + i * (uint32_t) sizeof(T)) = v;
 }
 
-// Streaming accesses (each element touched once per pass, as the stencil's
-// record reads and writes are): nontemporal for FP64 storage, whose record
-// rows of 16 cells fill whole 128-byte lines.  Measured on the 16^3
-// workload: FP64 stencil 5% faster; FP32 and BF16 rows are 64 / 32 bytes,
-// a line's halves are read by different instructions, and nontemporal
-// loads or stores made them 12-22% slower, so they stay cached.
+// Streaming accesses of the stencil (each element touched once per pass).
+// Loads are cached: in fused stages a block's face planes are read again by
+// its neighbours' workgroups, and cached FP64 loads made the stencil 4.4%
+// faster there (3.399 -> 3.248 ms on the 16^3 workload; nontemporal loads
+// had won by 5% before stages read faces in place).  Stores of new FP64
+// values are nontemporal (whole 128-byte record rows); FP32 and BF16 rows
+// are 64 / 32 bytes, a line's halves are written by different
+// instructions, and nontemporal stores made them 12-22% slower, so they
+// stay cached.
 template <typename T> __device__ __forceinline__ T ld_stream(gptr<T> base, uint32_t i)
 {
-   const gptr<T> p = (gptr<T>) ((gptr<char>) base + i * (uint32_t) sizeof(T));
-   if constexpr (sizeof(T) == 8)
-      return __builtin_nontemporal_load(p);
-   else
-      return *p;
+   return *(gptr<T>) ((gptr<char>) base + i * (uint32_t) sizeof(T));
 }
 template <typename T> __device__ __forceinline__ T ld_stream64(gptr<T> base, int64_t i)
 {
-   if constexpr (sizeof(T) == 8)
-      return __builtin_nontemporal_load(base + i);
-   else
-      return base[i];
+   return base[i];
 }
 template <typename T> __device__ __forceinline__ void st_stream(gptr<T> base, uint32_t i, T v)
 {
