@@ -227,6 +227,47 @@ int amr_dev_read_exchange_timing(amr_dev *dev, double ms[3]);
  * reference counterpart counts the program's own mallocs. */
 int amr_dev_memory(const amr_dev *dev, size_t *bytes, int *buffers);
 
+/* Refinement marking on the device: reset_all, check_objects (or the
+ * uniform rule) and the 2:1 balance fixpoint refine_level of one
+ * refinement step (ref/refine.c:66-82, 325-527; ref/move.c:65-623), in one
+ * launch with no host round trip between the fixpoint's passes.  Input:
+ * the active blocks in sorted order (level_start[l] = first block of level
+ * l, num_refine + 2 entries), the parent records and the objects.  Output:
+ * the refine flag (1 refine, 0 stay, -1 may coarsen) of every block and of
+ * every parent record with number >= 0 -- the flags the reference's
+ * sequential passes leave.  Returns AMR_ERR_ARG when the flags do not fit
+ * the kernel's local memory (n_blocks + n_parents > 65536) or an object
+ * type is unknown; the caller then marks on the host. */
+typedef struct {
+   int32_t type, pad;
+   double cen[3], size[3];
+} amr_mark_object;
+
+typedef struct {
+   int32_t level;
+   int32_t parent;        /* index into parents[], -1 for an initial block */
+   int32_t cen[3];        /* centre in units of the finest half-cell grid */
+   int32_t nei_level[6];  /* level of the face neighbour; -2 on the boundary */
+   int32_t nei[6];        /* index in blocks[] of a same-level or coarser
+                             face neighbour, else -1 */
+} amr_mark_block;
+
+typedef struct {
+   int32_t number;        /* < 0: unused record */
+   int32_t child[8];      /* index in blocks[], -1 when the child is a parent */
+} amr_mark_parent;
+
+typedef struct {
+   int32_t num_refine, cur_max_level, uniform_refine, refine_ghost;
+   int32_t mesh_size[3];  /* finest half-cell units per axis */
+   int32_t n_objects;
+} amr_mark_params;
+
+int amr_dev_mark_refine(amr_dev *dev, const amr_mark_params *p, const amr_mark_block *blocks,
+                        int n_blocks, const int32_t *level_start, const amr_mark_parent *parents,
+                        int n_parents, const amr_mark_object *objects, int8_t *block_refine,
+                        int8_t *parent_refine);
+
 #ifdef __cplusplus
 }
 #endif

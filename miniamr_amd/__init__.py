@@ -37,7 +37,7 @@ ABI_SYMBOLS = [
     "amr_dev_read_sums", "amr_dev_stage", "amr_dev_sync", "amr_dev_set_timing",
     "amr_dev_read_timing", "amr_dev_read_exchange_timing", "amr_dev_memory",
     "amr_dev_stage_fused", "amr_dev_set_materials", "amr_dev_stencil_stage",
-    "amr_dev_read_check_counts",
+    "amr_dev_read_check_counts", "amr_dev_mark_refine",
 ]
 
 
@@ -112,6 +112,7 @@ def load_library(path=LIB_PATH):
         "amr_dev_set_materials": (ip, [vp, ctypes.c_double, dp, ip]),
         "amr_dev_stencil_stage": (ip, [vp, ip, ip]),
         "amr_dev_read_check_counts": (ip, [vp, ctypes.POINTER(ctypes.c_int64)]),
+        "amr_dev_mark_refine": (ip, [vp, vp, vp, ip, i32p, vp, ip, vp, vp, vp]),
         # include/amr_batch.h, device half
         "amr_batch_create": (ip, [ip, ip, ip, ctypes.POINTER(vp)]),
         "amr_batch_destroy": (ip, [vp]),

@@ -119,6 +119,11 @@ struct amr_dev {
    unsigned long long *d_check_counts = nullptr;
    void *d_matwork = nullptr;
    size_t matwork_bytes = 0;
+   // amr_dev_mark_refine's inputs and output flags
+   void *mk_blocks = nullptr, *mk_parents = nullptr, *mk_objects = nullptr, *mk_levels = nullptr,
+        *mk_out = nullptr;
+   size_t mk_blocks_cap = 0, mk_parents_cap = 0, mk_objects_cap = 0, mk_levels_cap = 0,
+          mk_out_cap = 0;
    bool two_interiors = false;
    int fused_mode = 1;              // MINIAMR_FUSED: 0 never, 1 large stages, 2 always
    int oI_par[2] = {0, 0}, oZI_par[2] = {0, 0};

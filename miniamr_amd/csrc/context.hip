@@ -175,6 +175,11 @@ int amr_dev_destroy(amr_dev *d)
    hipFree(d->d_a0);
    hipFree(d->d_check_counts);
    hipFree(d->d_matwork);
+   hipFree(d->mk_blocks);
+   hipFree(d->mk_parents);
+   hipFree(d->mk_objects);
+   hipFree(d->mk_levels);
+   hipFree(d->mk_out);
    if (d->h_pinned)
       hipHostFree(d->h_pinned);
    if (d->stream && d->owns_stream)

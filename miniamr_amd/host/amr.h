@@ -46,6 +46,9 @@ typedef struct {
    int schedule;         /* 0: stage-major like the reference,
                             1: variable-major within a refinement interval */
    int device;
+   int device_marking;   /* MINIAMR_DEVICE_MARKING: -1 (unset) refinement
+                            marking on the device when it accepts the mesh,
+                            else on the host; 0 host; 1 device or fail */
    int seed;             /* --seed: generator seed of the initial values
                             (1 = the reference's implicit srand(1)) */
    int tape;             /* --tape: record data operations from allocation on */

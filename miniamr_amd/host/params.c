@@ -219,6 +219,8 @@ int amr_parse_args(amr_params *P, int argc, char **argv)
    P->schedule = (sch && !strcmp(sch, "var")) ? 1 : 0;
    if (P->stencil == 0)   /* coupled variables: keep the stage-major order */
       P->schedule = 0;
+   const char *mk = getenv("MINIAMR_DEVICE_MARKING");
+   P->device_marking = mk ? (atoi(mk) != 0) : -1;
    const char *dev = getenv("MINIAMR_DEVICE");
    P->device = dev ? atoi(dev) : 0;
    return 0;

@@ -152,6 +152,78 @@ int amr_refine_level(amr_run *R)
    return n;
 }
 
+/* reset_all + check_objects (or the uniform rule) + refine_level on the
+ * device (amr_dev_mark_refine): the same flags, one launch.  Returns 0 when
+ * the device declines (mesh too large for its local memory, an undefined
+ * object type, or a device library without it) and marking is not
+ * required; the caller then marks on the host. */
+static int device_mark(amr_run *R)
+{
+   const amr_params *P = &R->P;
+   if (P->device_marking == 0)
+      return 0;
+   const int na = amr_n_active(R), np = R->max_active_parent;
+   amr_mark_block *mb = (amr_mark_block *) amr_xmalloc(sizeof(amr_mark_block) * (na ? na : 1));
+   amr_mark_parent *mp = (amr_mark_parent *) amr_xmalloc(sizeof(amr_mark_parent) * (np ? np : 1));
+   amr_mark_object *mo =
+      (amr_mark_object *) amr_xmalloc(sizeof(amr_mark_object) * (P->num_objects ? P->num_objects : 1));
+   int32_t *index = (int32_t *) amr_xmalloc(sizeof(int32_t) * P->max_num_blocks);
+   int32_t *ls = (int32_t *) amr_xmalloc(sizeof(int32_t) * (P->num_refine + 2));
+   int8_t *bf = (int8_t *) amr_xmalloc(na ? na : 1), *pf = (int8_t *) amr_xmalloc(np ? np : 1);
+   for (int in = 0; in < na; in++)
+      index[R->sorted_slot[in]] = in;
+   for (int in = 0; in < na; in++) {
+      const amr_block *b = &R->blocks[R->sorted_slot[in]];
+      amr_mark_block *k = &mb[in];
+      k->level = b->level;
+      k->parent = b->parent;
+      for (int d = 0; d < 3; d++)
+         k->cen[d] = b->cen[d];
+      for (int f = 0; f < 6; f++) {
+         k->nei_level[f] = b->nei_level[f];
+         k->nei[f] = (b->nei_level[f] == b->level || b->nei_level[f] == b->level - 1)
+                        ? index[b->nei[f][0][0]] : -1;
+      }
+   }
+   for (int p = 0; p < np; p++) {
+      const amr_parent *pp = &R->parents[p];
+      mp[p].number = pp->number < 0 ? -1 : 0;
+      for (int c = 0; c < 8; c++)
+         mp[p].child[c] = (pp->number >= 0 && pp->child[c] >= 0) ? index[pp->child[c]] : -1;
+   }
+   for (int i = 0; i < P->num_objects; i++) {
+      const amr_object *o = &P->objects[i];
+      mo[i].type = o->type;
+      mo[i].pad = 0;
+      for (int d = 0; d < 3; d++) {
+         mo[i].cen[d] = o->cen[d];
+         mo[i].size[d] = o->size[d];
+      }
+   }
+   for (int l = 0; l <= P->num_refine + 1; l++)
+      ls[l] = R->sorted_index[l];
+   amr_mark_params mk;
+   mk.num_refine = P->num_refine;
+   mk.cur_max_level = R->cur_max_level;
+   mk.uniform_refine = P->uniform_refine;
+   mk.refine_ghost = P->refine_ghost;
+   for (int d = 0; d < 3; d++)
+      mk.mesh_size[d] = R->mesh_size[d];
+   mk.n_objects = P->num_objects;
+   int rc = amr_dev_mark_refine(R->dev, &mk, mb, na, ls, mp, np, mo, bf, pf);
+   if (rc == AMR_OK) {
+      for (int in = 0; in < na; in++)
+         R->blocks[R->sorted_slot[in]].refine = bf[in];
+      for (int p = 0; p < np; p++)
+         if (R->parents[p].number >= 0)
+            R->parents[p].refine = pf[p];
+   } else if (P->device_marking == 1) {
+      amr_fatal("ERROR: device mark_refine failed (%d): %s\n", rc, amr_dev_last_error(R->dev));
+   }
+   free(mb); free(mp); free(mo); free(index); free(ls); free(bf); free(pf);
+   return rc == AMR_OK;
+}
+
 /* Split every block marked 1, lowest level first (ref/block.c:41-343). */
 void amr_split_blocks(amr_run *R)
 {
@@ -380,19 +452,21 @@ void amr_refine(amr_run *R, int ts)
             R->cur_max_level = j;
             break;
          }
-      double t0 = amr_now();
-      reset_all(R);
-      if (P->uniform_refine) {
-         int na = amr_n_active(R);
-         for (int in = 0; in < na; in++) {
-            amr_block *b = &R->blocks[R->sorted_slot[in]];
-            b->refine = (b->level < P->num_refine) ? 1 : 0;
+      double t0 = amr_now(), t1 = t0;
+      if (!device_mark(R)) {
+         reset_all(R);
+         if (P->uniform_refine) {
+            int na = amr_n_active(R);
+            for (int in = 0; in < na; in++) {
+               amr_block *b = &R->blocks[R->sorted_slot[in]];
+               b->refine = (b->level < P->num_refine) ? 1 : 0;
+            }
+         } else {
+            amr_check_objects(R);
          }
-      } else {
-         amr_check_objects(R);
+         t1 = amr_now();
+         amr_refine_level(R);
       }
-      double t1 = amr_now();
-      amr_refine_level(R);
       double t2 = amr_now();
       amr_split_blocks(R);
       double t3 = amr_now();

@@ -493,6 +493,19 @@ int amr_dev_set_materials(amr_dev *d, double a1, const double *a0, int n)
    return AMR_OK;
 }
 
+/* The oracle leaves refinement marking to the host driver's own code
+ * (the reference's sequential passes); the device's one-launch marking is
+ * checked against the reference's runs directly. */
+int amr_dev_mark_refine(amr_dev *d, const amr_mark_params *p, const amr_mark_block *blocks,
+                        int n_blocks, const int32_t *level_start, const amr_mark_parent *parents,
+                        int n_parents, const amr_mark_object *objects, int8_t *block_refine,
+                        int8_t *parent_refine)
+{
+   (void) p; (void) blocks; (void) n_blocks; (void) level_start; (void) parents;
+   (void) n_parents; (void) objects; (void) block_refine; (void) parent_refine;
+   return fail(d, AMR_ERR_STATE, "the CPU oracle marks refinement on the host");
+}
+
 int amr_dev_read_check_counts(amr_dev *d, int64_t counts[2])
 {
    counts[0] = d->checks[0];

@@ -178,10 +178,29 @@ def test_driver_matches_reference(name):
     kind = "cells" if "cells" in g else ("bsums" if "bsums" in g else None)
     if kind == "bsums" and "vars" in g["bsums"]:
         kind = "bsums_only"        # large mesh: per-block sums only
-    r = A.run(M.EXE_PATH, g["args"], dump=kind, timeout=110)
+    # refinement marked on the device (fails loudly if the device declines)
+    r = A.run(M.EXE_PATH, g["args"], dump=kind, timeout=110,
+              env={"MINIAMR_DEVICE_MARKING": "1"})
     check_cells = not any("difference too large" in m for m in g["messages"])
     rep = A.compare(r, g, sum_rtol=1e-12, max_ulp=4, check_cells=check_cells)
     print(name, rep)
+
+
+# refine_ghost, every object family, three levels, uniform refinement:
+# the host's sequential marking and the device's one-launch marking agree
+@pytest.mark.parametrize("name", ["t27_cv", "t7_r3", "obj02", "obj13", "obj21", "obj24",
+                                  "t27_uniform", "t27_perm"])
+def test_host_and_device_marking_agree(name):
+    require(M.EXE_PATH, "run __graft_entry__.build()")
+    g = A.golden(name)
+    print("case: marking %s" % name, flush=True)
+    a = A.run(M.EXE_PATH, g["args"], dump="cells", timeout=110,
+              env={"MINIAMR_DEVICE_MARKING": "0"})
+    b = A.run(M.EXE_PATH, g["args"], dump="cells", timeout=110,
+              env={"MINIAMR_DEVICE_MARKING": "1"})
+    assert a["rc"] == 0 and b["rc"] == 0
+    assert a["counts"] == b["counts"] and a["sums"] == b["sums"]
+    assert np.array_equal(a["cells"]["cells"], b["cells"]["cells"])
 
 
 @pytest.mark.parametrize("name", ["t27_perm", "t27_cv", "t7_r3", "cfg4"])
