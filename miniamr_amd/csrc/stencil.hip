@@ -138,20 +138,23 @@ __global__ void k_convert_out(const T *__restrict__ pool, double *__restrict__ s
    }
 }
 
-// Variables per workgroup: two for the narrow storage types (the per-cell
-// index work and barriers are shared), one for FP64.  MINIAMR_STENCIL_VPW=4
-// selects four for the narrow types.
-template <typename T> struct Vpw { static constexpr int v = 2; };
+// Variables per workgroup: one for FP64; for the narrow storage types the
+// per-cell index work and barriers are shared by several: two for FP32,
+// four for BF16 (16^3 workload, fused stages: BF16 1.535 -> 1.443 ms per
+// launch from two to four, FP32 1.825 -> 1.805, both ~40% slower with
+// one).  MINIAMR_STENCIL_VPW=1|2|4 overrides the narrow types' choice.
+template <typename T> struct Vpw { static constexpr int v = sizeof(T) == 2 ? 4 : 2; };
 template <> struct Vpw<double> { static constexpr int v = 1; };
 
+template <typename T>
 static int narrow_vpw()
 {
    static int v = -1;
    if (v < 0) {
       const char *e = getenv("MINIAMR_STENCIL_VPW");
-      v = e ? atoi(e) : 2;
-      if (v != 1 && v != 4)
-         v = 2;
+      v = e ? atoi(e) : Vpw<T>::v;
+      if (v != 1 && v != 2 && v != 4)
+         v = Vpw<T>::v;
    }
    return v;
 }
@@ -206,9 +209,9 @@ int launch_st(amr_dev *d, int start, int num, double *parts)
 {
    if (Vpw<T>::v == 1)
       return launch_st_v<T, ST, 1>(d, start, num, parts);
-   if (narrow_vpw() == 4)
+   if (narrow_vpw<T>() == 4)
       return launch_st_v<T, ST, 4>(d, start, num, parts);
-   if (narrow_vpw() == 1)
+   if (narrow_vpw<T>() == 1)
       return launch_st_v<T, ST, 1>(d, start, num, parts);
    return launch_st_v<T, ST, 2>(d, start, num, parts);
 }
@@ -223,10 +226,9 @@ int launch_stencil(amr_dev *d, int start, int num, bool want_partials)
                             : launch_st<T, 27>(d, start, num, parts);
 }
 
-template <typename T, int ST>
-int launch_fused(amr_dev *d, int start, int num, double *parts)
+template <typename T, int ST, int V>
+int launch_fused_v(amr_dev *d, int start, int num, double *parts)
 {
-   constexpr int V = Vpw<T>::v;
    Lay L;
    if (int rc = amr_lay(d, start, num, &L))
       return rc;
@@ -256,6 +258,19 @@ int launch_fused(amr_dev *d, int start, int num, double *parts)
    for (int v = start; v < start + num; v++)
       d->par[v] ^= 1;
    return AMR_OK;
+}
+
+// variables per workgroup as for the plain stencil (MINIAMR_STENCIL_VPW)
+template <typename T, int ST>
+int launch_fused(amr_dev *d, int start, int num, double *parts)
+{
+   if (Vpw<T>::v == 1)
+      return launch_fused_v<T, ST, 1>(d, start, num, parts);
+   if (narrow_vpw<T>() == 4)
+      return launch_fused_v<T, ST, 4>(d, start, num, parts);
+   if (narrow_vpw<T>() == 1)
+      return launch_fused_v<T, ST, 1>(d, start, num, parts);
+   return launch_fused_v<T, ST, 2>(d, start, num, parts);
 }
 
 template <typename T>
