@@ -18,10 +18,16 @@
  *     on_proc_comm      ref/comm.c:1465-1580      (face jobs, kind SAME)
  *     on_proc_comm_diff ref/comm.c:1585-1908      (kinds PROLONG, RESTRICT)
  *     apply_bc          ref/comm.c:1911-1965      (kind BC)
- *   stencil_driver(var,stage) ref/stencil.c:43-145  amr_dev_stencil
+ *   stencil_driver(var,stage) ref/stencil.c:43-981  amr_dev_stencil_stage
+ *                                                 (amr_dev_stencil: stage 0;
+ *                                                 --stencil 0 coefficients of
+ *                                                 ref/init.c:418-423:
+ *                                                 amr_dev_set_materials)
  *   check_sum(var)      ref/check_sum.c:36-65    amr_dev_check_sum
  *   split_blocks() data loop ref/block.c:161-173   amr_dev_split
  *   consolidate_blocks() data loop ref/block.c:411-431  amr_dev_consolidate
+ *   reset_all + check_objects + refine_level     amr_dev_mark_refine
+ *                       ref/refine.c:66-82, 325-527, ref/move.c:65-623
  *
  * Conventions
  *   - Cell arrays use the reference's index order array[var][i][j][k]
