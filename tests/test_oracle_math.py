@@ -14,7 +14,7 @@ import amrcheck as A
 import meshgen
 from conftest import require
 from miniamr_amd import Device, load_library, FACE_JOB, FACE_SAME, FACE_PROLONG, \
-    FACE_RESTRICT, FACE_BC
+    FACE_RESTRICT, FACE_BC, FACE_GHOSTS
 
 
 @pytest.fixture(scope="module")
@@ -87,6 +87,50 @@ def test_prolong_restrict_closed_form(olib):
         untouched = np.ones((ny + 2, nz + 2), bool)
         untouched[1:hy + 1, hz + 1:nz + 1] = False
         np.testing.assert_array_equal(c_out[v, 0][untouched], coarse[v, 0][untouched])
+
+
+def test_whole_plane_coarse_fine_closed_forms(olib):
+    """--code 1/2 coarse/fine jobs over whole ghost planes: a coarse block
+    (slot 0) with four fine blocks (slots 1-4) on its west face.  Constant
+    fields: every fine ghost cell, ring included, is coarse / 4; the coarse
+    ghost plane is covered exactly once by the four quarters, with 4 fine
+    cells per interior cell, 2 per ring edge cell and 1 per corner.  A field
+    linear in y pins which fine rows pair up."""
+    nx, ny, nz = 4, 6, 8
+    d = dev(olib, nx, ny, nz, nv=2, st=7)
+    quarters = [(iq, jq) for jq in range(2) for iq in range(2)]
+    pro = [(FACE_PROLONG | FACE_GHOSTS, 1, 1 + q, 0, iq, jq) for q, (iq, jq) in enumerate(quarters)]
+    res = [(FACE_RESTRICT | FACE_GHOSTS, 0, 0, 1 + q, iq, jq) for q, (iq, jq) in enumerate(quarters)]
+    jobs = [np.array(pro + res, dtype=FACE_JOB), np.zeros(0, FACE_JOB), np.zeros(0, FACE_JOB)]
+    d.set_mesh([0, 1, 2, 3, 4], jobs)
+    d.upload(0, np.full(d.shape, 8.0))
+    for s in range(1, 5):
+        d.upload(s, np.full(d.shape, 1.0))
+    d.comm(0, 2)
+    for s in range(1, 5):
+        np.testing.assert_array_equal(d.download(s)[:, nx + 1], 2.0)
+    g = d.download(0)[:, 0]
+    want = np.full((ny + 2, nz + 2), 4.0)
+    want[[0, -1], :] = 2.0
+    want[:, [0, -1]] = 2.0
+    want[0, 0] = want[0, -1] = want[-1, 0] = want[-1, -1] = 1.0
+    for v in range(2):
+        np.testing.assert_array_equal(g[v], want)
+    # fine cells = y index j: an interior coarse row j' of a quarter sums
+    # fine rows 2j'-1 and 2j' twice (8j' - 2); the ring rows hold 2*0 and
+    # 2*(ny+1); ring columns 2j'-1 + 2j'
+    f = np.broadcast_to(np.arange(ny + 2, dtype=float)[None, None, :, None], d.shape).copy()
+    for s in range(1, 5):
+        d.upload(s, f)
+    d.comm(0, 2)
+    g = d.download(0)[0, 0]
+    h = ny // 2
+    for j in range(1, ny + 1):
+        jp = j - (h if j > h else 0)
+        np.testing.assert_array_equal(g[j, 1:-1], 8.0 * jp - 2.0)
+        assert g[j, 0] == g[j, -1] == 4.0 * jp - 1.0
+    np.testing.assert_array_equal(g[0, 1:-1], 0.0)
+    np.testing.assert_array_equal(g[-1, 1:-1], 2.0 * (ny + 1))
 
 
 @pytest.mark.parametrize("st", [7, 27])
