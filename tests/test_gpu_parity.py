@@ -265,3 +265,23 @@ def test_precision_form_new_modes(prec, name):
     e = A.rel_err(r["sums"][::nv], g["sums"][::nv])
     print("%s %s: variable 0 max checksum rel err %.3e" % (prec, name, e.max()))
     assert e.max() <= (1e-6 if prec == "f32" else 5e-2)
+
+
+def test_code1_bandwidth_mesh_matches_oracle():
+    """--code 1 (whole ghost planes) at the bandwidth-scale mesh: 4x4x4
+    blocks of 16^3, four levels, two moving objects (about 4,000 active
+    blocks), 2 variables, 4 timesteps -- checksums and per-level block
+    counts against the CPU oracle."""
+    require(M.EXE_PATH, "run __graft_entry__.build()")
+    require(A.ORACLE_X, "run __graft_entry__.build()")
+    args = ("--init_x 4 --init_y 4 --init_z 4 --nx 16 --ny 16 --nz 16 --code 1 --num_vars 2 "
+            "--stencil 27 --num_tsteps 4 --stages_per_ts 6 --checksum_freq 3 --num_refine 4 "
+            "--refine_freq 2 --max_blocks 8000 --error_tol -10 --num_objects 2 "
+            "--object 2 0 0.15 0.15 0.15 0.02 0.02 0.02 0.1 0.1 0.1 0 0 0 "
+            "--object 0 0 0.5 0.5 0.5 0.02 0.02 0.02 0.05 0.05 0.05 0 0 0")
+    print("case: code 1 bandwidth mesh", flush=True)
+    g = A.run(M.EXE_PATH, args, timeout=100)
+    c = A.run(A.ORACLE_X, args, timeout=100)
+    assert g["rc"] == 0 and c["rc"] == 0, (g["stdout"][-1000:], c["stdout"][-1000:])
+    assert g["counts"] == c["counts"] and len(g["sums"]) == len(c["sums"])
+    assert A.rel_err(g["sums"], c["sums"]).max() <= 1e-12
