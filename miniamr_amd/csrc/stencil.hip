@@ -138,16 +138,16 @@ __global__ void k_convert_out(const T *__restrict__ pool, double *__restrict__ s
    }
 }
 
-// Variables per workgroup: one for FP64; for the narrow storage types the
-// per-cell index work and barriers are shared by several: two for FP32,
-// four for BF16 (16^3 workload, fused stages: BF16 1.535 -> 1.443 ms per
-// launch from two to four, FP32 1.825 -> 1.805, both ~40% slower with
-// one).  MINIAMR_STENCIL_VPW=1|2|4 overrides the narrow types' choice.
+// Variables per workgroup: the per-cell index work, plane barriers and
+// ring bookkeeping are shared by several variables.  Measured on the 16^3
+// workload, fused stages, ms per launch for 1 / 2 / 4 variables: FP64
+// 3.240 / 3.093 / 3.170, FP32 2.537 / 1.825 / 1.805, BF16 2.166 / 1.535 /
+// 1.443; so two for FP64 and FP32, four for BF16.  MINIAMR_STENCIL_VPW=
+// 1|2|4 overrides the choice.
 template <typename T> struct Vpw { static constexpr int v = sizeof(T) == 2 ? 4 : 2; };
-template <> struct Vpw<double> { static constexpr int v = 1; };
 
 template <typename T>
-static int narrow_vpw()
+static int stencil_vpw()
 {
    static int v = -1;
    if (v < 0) {
@@ -207,11 +207,9 @@ int launch_st_v(amr_dev *d, int start, int num, double *parts)
 template <typename T, int ST>
 int launch_st(amr_dev *d, int start, int num, double *parts)
 {
-   if (Vpw<T>::v == 1)
-      return launch_st_v<T, ST, 1>(d, start, num, parts);
-   if (narrow_vpw<T>() == 4)
+   if (stencil_vpw<T>() == 4)
       return launch_st_v<T, ST, 4>(d, start, num, parts);
-   if (narrow_vpw<T>() == 1)
+   if (stencil_vpw<T>() == 1)
       return launch_st_v<T, ST, 1>(d, start, num, parts);
    return launch_st_v<T, ST, 2>(d, start, num, parts);
 }
@@ -264,11 +262,9 @@ int launch_fused_v(amr_dev *d, int start, int num, double *parts)
 template <typename T, int ST>
 int launch_fused(amr_dev *d, int start, int num, double *parts)
 {
-   if (Vpw<T>::v == 1)
-      return launch_fused_v<T, ST, 1>(d, start, num, parts);
-   if (narrow_vpw<T>() == 4)
+   if (stencil_vpw<T>() == 4)
       return launch_fused_v<T, ST, 4>(d, start, num, parts);
-   if (narrow_vpw<T>() == 1)
+   if (stencil_vpw<T>() == 1)
       return launch_fused_v<T, ST, 1>(d, start, num, parts);
    return launch_fused_v<T, ST, 2>(d, start, num, parts);
 }
