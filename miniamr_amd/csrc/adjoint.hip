@@ -315,16 +315,20 @@ __device__ __forceinline__ void adj_exchange_job(T *__restrict__ lam, const amr_
       return (k == 1 || k == L.nz) ? zcopy(L, i, j, k) : -1;
    };
    const T zero = from_acc<T>(A(0));
-   if (jb.kind == AMR_FACE_SAME || jb.kind == AMR_FACE_BC) {
+   const int kind = jb.kind & 3;                 // AMR_FACE_SAME .. AMR_FACE_BC
+   const bool whole = jb.kind & AMR_FACE_GHOSTS;  // --code 1/2: ghost ring included
+   if (kind == AMR_FACE_SAME || kind == AMR_FACE_BC) {
       int ext_a = 0, ext_b = 0;
-      if (L.stencil == 27) {
-         if (jb.kind == AMR_FACE_BC) ext_a = ext_b = 1;
+      if (whole) {
+         ext_a = ext_b = 1;
+      } else if (L.stencil == 27) {
+         if (kind == AMR_FACE_BC) ext_a = ext_b = 1;
          else if (dir == 1) ext_a = 1;
          else if (dir == 2) ext_a = ext_b = 1;
       }
       const int a0 = 1 - ext_a, b0 = 1 - ext_b;
       const int wa = na + 2 * ext_a, wb = nb + 2 * ext_b;
-      const int sp = jb.kind == AMR_FACE_BC ? (hi ? n : 1) : plane;
+      const int sp = kind == AMR_FACE_BC ? (hi ? n : 1) : plane;
       for (int t = tid; t < wa * wb; t += nthr) {
          const int a = a0 + t / wb, b = b0 + t % wb;
          int i, j, k, si, sj, sk;
@@ -348,7 +352,111 @@ __device__ __forceinline__ void adj_exchange_job(T *__restrict__ lam, const amr_
                else st(sq(q), so, from_acc<T>(sv[q] + mu[q]));
             }
       }
-   } else if (jb.kind == AMR_FACE_PROLONG) {
+   } else if (kind == AMR_FACE_PROLONG && whole) {
+      // whole fine plane: coarse cell (ca, cb) of [ao, ao+ha+1] x
+      // [bo, bo+hb+1] fed the fine ghosts (a, b) with ao + (a+1)/2 == ca:
+      // fine row 0 for ca == ao, na+1 for ca == ao+ha+1, else 2(ca-ao)-1
+      // and 2(ca-ao); columns alike.  Neighbouring quarters share the ring
+      // rows of the coarse plane, so such plans take the atomic path.
+      const int ha = na / 2, hb = nb / 2;
+      const int ao = jb.jq * ha, bo = jb.iq * hb;
+      const int wb = hb + 2;
+      for (int t = tid; t < (ha + 2) * wb; t += nthr) {
+         const int ca = ao + t / wb, cb = bo + t % wb;
+         const int ra = ca - ao, rb = cb - bo;      // 0 .. h+1
+         const int fa0 = ra == 0 ? 0 : ra == ha + 1 ? na + 1 : 2 * ra - 1;
+         const int fb0 = rb == 0 ? 0 : rb == hb + 1 ? nb + 1 : 2 * rb - 1;
+         const int nfa = (ra == 0 || ra == ha + 1) ? 1 : 2, nfb = (rb == 0 || rb == hb + 1) ? 1 : 2;
+         int dh[4], dc[4];
+#pragma unroll
+         for (int r = 0; r < 4; r++) {
+            const int rr = (r >> 1) < nfa && (r & 1) < nfb ? r : 0;
+            int i, j, k;
+            canon(dir, ghost, fa0 + (rr >> 1), fb0 + (rr & 1), i, j, k);
+            dh[r] = (r == rr) ? home(L, i, j, k) : -1;
+            dc[r] = (r == rr) ? dstcopy(i, j, k) : -1;
+         }
+         int si, sj, sk;
+         canon(dir, plane, ca, cb, si, sj, sk);
+         const int so = srcoff(si, sj, sk);
+         A mu[KV], sv[KV];
+#pragma unroll
+         for (int q = 0; q < KV; q++)
+            if (q < nv) {
+               A m = 0;
+#pragma unroll
+               for (int r = 0; r < 4; r++) {
+                  if (dh[r] < 0)
+                     continue;
+                  m = m + to_acc(ld(dq(q), dh[r]));
+                  if (dc[r] >= 0) m = m + to_acc(ld(dq(q), dc[r]));
+               }
+               mu[q] = m / A(4);
+               if (!ATOMIC) sv[q] = to_acc(ld(sq(q), so));
+            }
+#pragma unroll
+         for (int q = 0; q < KV; q++)
+            if (q < nv) {
+#pragma unroll
+               for (int r = 0; r < 4; r++) {
+                  if (dh[r] < 0)
+                     continue;
+                  st(dq(q), dh[r], zero);
+                  if (dc[r] >= 0) st(dq(q), dc[r], zero);
+               }
+               if (ATOMIC) atomicAdd(&src[q * vs + so], mu[q]);
+               else st(sq(q), so, from_acc<T>(sv[q] + mu[q]));
+            }
+      }
+   } else if (kind == AMR_FACE_RESTRICT && whole) {
+      // the quarter plus its ring share: each coarse ghost was the sum of
+      // 4 fine cells (interior), 2 fine ring cells (ring edge) or 1 (corner)
+      const int ha = na / 2, hb = nb / 2;
+      const int ao = jb.jq * ha, bo = jb.iq * hb;
+      const int a_lo = jb.jq ? ha + 1 : 0, b_lo = jb.iq ? hb + 1 : 0, wb = hb + 1;
+      for (int t = tid; t < (ha + 1) * wb; t += nthr) {
+         const int a = a_lo + t / wb, b = b_lo + t % wb;
+         const bool ra = a == 0 || a == na + 1, rb = b == 0 || b == nb + 1;
+         const int fa = ra ? a : 2 * (a - ao) - 1, fb = rb ? b : 2 * (b - bo) - 1;
+         const int nfb = rb ? 1 : 2, cnt = (ra ? 1 : 2) * nfb;
+         int i, j, k;
+         canon(dir, ghost, a, b, i, j, k);
+         const int dh = home(L, i, j, k), dc = dstcopy(i, j, k);
+         int so[4];
+#pragma unroll
+         for (int r = 0; r < 4; r++) {
+            const int rr = r < cnt ? r : 0;
+            int si, sj, sk;
+            canon(dir, plane, fa + rr / nfb, fb + rr % nfb, si, sj, sk);
+            so[r] = srcoff(si, sj, sk);
+         }
+         A mu[KV], sv[KV][4];
+#pragma unroll
+         for (int q = 0; q < KV; q++)
+            if (q < nv) {
+               mu[q] = to_acc(ld(dq(q), dh));
+               if (dc >= 0) mu[q] = mu[q] + to_acc(ld(dq(q), dc));
+               if (!ATOMIC) {
+#pragma unroll
+                  for (int r = 0; r < 4; r++)
+                     if (r < cnt) sv[q][r] = to_acc(ld(sq(q), so[r]));
+               }
+            }
+#pragma unroll
+         for (int q = 0; q < KV; q++)
+            if (q < nv) {
+               st(dq(q), dh, zero);
+               if (dc >= 0) st(dq(q), dc, zero);
+#pragma unroll
+               for (int r = 0; r < 4; r++) {
+                  if (r >= cnt)
+                     continue;
+                  if (ATOMIC) atomicAdd(&src[q * vs + so[r]], mu[q]);
+                  else st(sq(q), so[r], from_acc<T>(sv[q][r] + mu[q]));
+               }
+            }
+      }
+   } else if (kind == AMR_FACE_PROLONG) {
       // each coarse cell of the quarter fed 4 fine ghosts with value / 4
       const int ha = na / 2, hb = nb / 2;
       const int ao = jb.jq * ha, bo = jb.iq * hb;
@@ -564,15 +672,17 @@ void exclusive_reads(amr_adj *a, const TapePlan &pl)
       bool ok = true;
       for (int t = off; t < off + pl.count[dir] && ok; t++) {
          const amr_face_job &jb = pl.jobs[t];
-         const int hi = jb.face & 1;
+         const int hi = jb.face & 1, kind = jb.kind & 3;
          int slot, side, mask;
-         if (jb.kind == AMR_FACE_BC) {
+         if (kind == AMR_FACE_BC) {
             slot = jb.dst;
             side = hi;        // the block's own plane next to the face
             mask = 16;
          } else {
             slot = jb.src;
             side = !hi;       // the neighbour's plane next to the shared face
+            // a whole-plane (--code 1/2) prolongation reads beyond its
+            // quarter (the ring rows shared with the neighbouring quarters)
             mask = jb.kind == AMR_FACE_PROLONG ? 1 << (jb.iq * 2 + jb.jq) : 16;
          }
          int &m = seen[{slot, side}];
@@ -762,8 +872,6 @@ int amr_dev_tape(amr_dev *d, int on)
    if (!d)
       return AMR_ERR_ARG;
    if (on) {
-      if (d->whole_ghosts)
-         return amr_set_error(d, AMR_ERR_ARG, "tape: the adjoint covers the --code 0 exchange only");
       if (d->g.stencil == 0)
          return amr_set_error(d, AMR_ERR_ARG, "tape: the adjoint covers the 7/27-point stencils");
       delete d->tape;

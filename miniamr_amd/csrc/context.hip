@@ -438,9 +438,6 @@ int amr_dev_set_mesh(amr_dev *d, const int32_t *sorted, int n_active,
    int rc = amr_set_mesh_impl(d, sorted, n_active, jobs, count);
    if (rc || !d->tape || !d->tape->on || d->replaying)
       return rc;
-   if (d->whole_ghosts)
-      return amr_set_error(d, AMR_ERR_ARG,
-                           "set_mesh: the tape covers the --code 0 exchange only");
    TapePlan pl;
    pl.sorted.assign(sorted, sorted + n_active);
    pl.jobs.assign(jobs, jobs + (count[0] + count[1] + count[2]));

@@ -18,7 +18,11 @@ step restates the reference's arithmetic:
                   boundary: copy of the adjacent plane    ref/comm.c:1911-1965
                 with the 27-point stencil's extended in-plane ranges
                 (ref/comm.c:1496-1533, 1941-1964) and the direction order
-                of --permute (ref/comm.c:45-55);
+                of --permute (ref/comm.c:45-55); --code 1/2 fills (kind |
+                GHOSTS) cover whole ghost planes: prolongation over the ring
+                too, restriction of a quarter plus its share of the ring
+                (2 fine ring cells per ring edge cell, 1 at the corner)
+                                                          ref/comm.c:1535-1579, 1689-1907;
   stencil       7-point (((((W+S)+D)+C)+U)+N)+E, / 7      ref/stencil.c:82-101
                 27-point ((sb + sm) + sf) / 27            ref/stencil.c:102-144
 Only what the tape holds is used; values come from the caller's tensor.
@@ -32,6 +36,7 @@ import torch
 
 OP_UPLOAD, OP_SET_MESH, OP_SPLIT, OP_MERGE, OP_COMM, OP_STENCIL, OP_STAGE = range(1, 8)
 SAME, PROLONG, RESTRICT, BC = 0, 1, 2, 3
+GHOSTS = 4   # or'ed into a kind: whole ghost planes (--code 1/2)
 PERM = [(0, 1, 2), (1, 2, 0), (2, 0, 1), (0, 2, 1), (1, 0, 2), (2, 1, 0)]
 
 
@@ -91,14 +96,19 @@ class Restatement:
         na = self.ny if d == 0 else self.nx
         nb = self.ny if d == 2 else self.nz
         cp_d, cp_s, pr_d, pr_s, rs_d, rs_s = [], [], [], [], [], []
+        r1_d, r1_s, r2_d, r2_s = [], [], [], []   # ring cells of whole-plane restrictions
         for jb in fj[off:off + cnt[d]]:
             kind, face, dst, src, iq, jq = (int(x) for x in jb)
+            whole = kind & GHOSTS
+            kind &= 3
             hi = face & 1
             ghost = n + 1 if hi else 0
             plane = 1 if hi else n
             if kind in (SAME, BC):
                 ea = eb = 0
-                if self.stencil == 27:
+                if whole:
+                    ea = eb = 1
+                elif self.stencil == 27:
                     if kind == BC:
                         ea = eb = 1
                     elif d == 1:
@@ -113,11 +123,32 @@ class Restatement:
                         cp_s.append(self.flat(s_slot, *_canon(d, sp, a, b)))
             elif kind == PROLONG:
                 ao, bo = jq * (na // 2), iq * (nb // 2)
-                for a in range(1, na + 1):
-                    for b in range(1, nb + 1):
+                w = 1 if whole else 0
+                for a in range(1 - w, na + 1 + w):
+                    for b in range(1 - w, nb + 1 + w):
                         ca, cb = (a + 1) // 2 + ao, (b + 1) // 2 + bo
                         pr_d.append(self.flat(dst, *_canon(d, ghost, a, b)))
                         pr_s.append(self.flat(src, *_canon(d, plane, ca, cb)))
+            elif whole:
+                ha, hb = na // 2, nb // 2
+                ao, bo = jq * ha, iq * hb
+                arange = range(ha + 1, na + 2) if jq else range(0, ha + 1)
+                brange = range(hb + 1, nb + 2) if iq else range(0, hb + 1)
+                for a in arange:
+                    fa = [a] if a in (0, na + 1) else [2 * (a - ao) - 1, 2 * (a - ao)]
+                    for b in brange:
+                        fb = [b] if b in (0, nb + 1) else [2 * (b - bo) - 1, 2 * (b - bo)]
+                        terms = [self.flat(src, *_canon(d, plane, x, y)) for x in fa for y in fb]
+                        dcell = self.flat(dst, *_canon(d, ghost, a, b))
+                        if len(terms) == 4:
+                            rs_d.append(dcell)
+                            rs_s.append(terms)
+                        elif len(terms) == 2:
+                            r2_d.append(dcell)
+                            r2_s.append(terms)
+                        else:
+                            r1_d.append(dcell)
+                            r1_s.append(terms[0])
             else:
                 ha, hb = na // 2, nb // 2
                 ao, bo = jq * ha, iq * hb
@@ -129,18 +160,21 @@ class Restatement:
                                      for r in range(4)])
         t = lambda x: torch.tensor(x, dtype=torch.long)
         res = (t(cp_d), t(cp_s), t(pr_d), t(pr_s), t(rs_d),
-               t(rs_s).reshape(-1, 4) if rs_s else torch.zeros((0, 4), dtype=torch.long))
+               t(rs_s).reshape(-1, 4) if rs_s else torch.zeros((0, 4), dtype=torch.long),
+               t(r1_d), t(r1_s), t(r2_d),
+               t(r2_s).reshape(-1, 2) if r2_s else torch.zeros((0, 2), dtype=torch.long))
         self._fills[key] = res
         return res
 
     def exchange(self, u, plan, d, v0, v1):
-        cp_d, cp_s, pr_d, pr_s, rs_d, rs_s = self.fills(plan, d)
+        cp_d, cp_s, pr_d, pr_s, rs_d, rs_s, r1_d, r1_s, r2_d, r2_s = self.fills(plan, d)
         f = u.reshape(self.V, -1)
         rows = f[v0:v1]
-        dst = torch.cat([cp_d, pr_d, rs_d])
+        dst = torch.cat([cp_d, pr_d, rs_d, r1_d, r2_d])
         vals = torch.cat([rows[:, cp_s], rows[:, pr_s] / 4.0,
                           ((rows[:, rs_s[:, 0]] + rows[:, rs_s[:, 1]]) + rows[:, rs_s[:, 2]])
-                          + rows[:, rs_s[:, 3]]], dim=1)
+                          + rows[:, rs_s[:, 3]],
+                          rows[:, r1_s], rows[:, r2_s[:, 0]] + rows[:, r2_s[:, 1]]], dim=1)
         rows = rows.index_copy(1, dst, vals)
         f = torch.cat([f[:v0], rows, f[v1:]])
         return f.reshape(u.shape)

@@ -12,7 +12,8 @@ import miniamr_amd as M
 OP_UPLOAD, OP_SET_MESH, OP_SPLIT, OP_MERGE, OP_COMM, OP_STENCIL, OP_STAGE = range(1, 8)
 
 
-def build(seed, n=(4, 6, 8), V=3, stencil=27):
+def build(seed, n=(4, 6, 8), V=3, stencil=27, ghosts=False):
+    """ghosts: the plans' jobs move whole ghost planes (--code 1/2)."""
     rng = np.random.default_rng(seed)
     nparents = 2
     n_split = meshgen.split_slots_needed(nparents)
@@ -21,8 +22,8 @@ def build(seed, n=(4, 6, 8), V=3, stencil=27):
     G = (V, n[0] + 2, n[1] + 2, n[2] + 2)
     uploads = list(range(slots))
     u0 = rng.random((slots,) + G)                 # whole blocks, ghosts included
-    plan0 = meshgen.random_plan(rng, n_src, n_dst)
-    plan1 = meshgen.random_plan(rng, n_src, n_dst)
+    plan0 = meshgen.random_plan(rng, n_src, n_dst, ghosts=ghosts)
+    plan1 = meshgen.random_plan(rng, n_src, n_dst, ghosts=ghosts)
     plans = {}
     for p, pl in enumerate((plan0, plan1)):
         jobs = np.concatenate(pl)

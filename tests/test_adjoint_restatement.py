@@ -27,10 +27,11 @@ def restated(t):
                           c["stencil"], c["num_vars"], c["max_blocks"])
 
 
+@pytest.mark.parametrize("ghosts", [False, True], ids=["code0", "code1"])
 @pytest.mark.parametrize("stencil", [7, 27])
-def test_restatement_matches_c_oracle(stencil):
+def test_restatement_matches_c_oracle(stencil, ghosts):
     require(A.ORACLE_SO, "run __graft_entry__.build()")
-    t = tapegen.build(5 + stencil, stencil=stencil)
+    t = tapegen.build(5 + stencil, stencil=stencil, ghosts=ghosts)
     dev = M.Device(lib=M.load_library(A.ORACLE_SO), **t["cfg"])
     tapegen.apply(dev, t)
     u, _ = restated(t).run(torch.from_numpy(t["u0"]), t["uploads"])

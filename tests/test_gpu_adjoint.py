@@ -28,7 +28,6 @@ import pytest
 import torch
 
 import amrcheck as A
-import meshgen
 import miniamr_amd as M
 import tapegen
 from conftest import require
@@ -49,11 +48,14 @@ def _lib():
     return M.load_library(M.LIB_PATH)
 
 
+# code 1: the plans' jobs move whole ghost planes (--code 1/2); their
+# prolongations share coarse ring rows, so the adjoint takes its atomic path
+@pytest.mark.parametrize("ghosts", [False, True], ids=["code0", "code1"])
 @pytest.mark.parametrize("stencil", [7, 27])
-def test_synthetic_tape_gradient_matches_autograd(stencil):
-    print("case: synthetic tape stencil=%d" % stencil, flush=True)
+def test_synthetic_tape_gradient_matches_autograd(stencil, ghosts):
+    print("case: synthetic tape stencil=%d ghosts=%s" % (stencil, ghosts), flush=True)
     lib = _lib()
-    t = tapegen.build(50 + stencil, stencil=stencil)
+    t = tapegen.build(50 + stencil, stencil=stencil, ghosts=ghosts)
     dev = M.Device(lib=lib, **t["cfg"])
     assert lib.amr_dev_tape(dev.h, 1) == 0
     tapegen.apply(dev, t)
@@ -235,18 +237,9 @@ def test_fp32_adjoint_tracks_fp64(taped):
         tr32.close()
 
 
-def test_tape_refuses_whole_ghost_plans():
-    """The adjoint covers the --code 0 exchange: taping a --code 1/2 plan
-    (whole ghost planes) is refused at tape start and at set_mesh."""
+def test_tape_refuses_stencil0():
+    """The adjoint kernels cover the 7/27-point stencils: taping a
+    --stencil 0 context is refused."""
     lib = _lib()
-    rng = np.random.default_rng(3)
-    kw = dict(nx=4, ny=4, nz=4, num_vars=1, max_blocks=8, stencil=7)
-    dev = M.Device(lib=lib, **kw)
-    order = list(range(6))
-    dev.set_mesh(order, meshgen.random_plan(rng, 2, 4, ghosts=True))
+    dev = M.Device(lib=lib, nx=4, ny=4, nz=4, num_vars=8, max_blocks=4, stencil=0)
     assert lib.amr_dev_tape(dev.h, 1) == -1   # AMR_ERR_ARG
-    dev2 = M.Device(lib=lib, **kw)
-    assert lib.amr_dev_tape(dev2.h, 1) == 0
-    with pytest.raises(M.AmrError):
-        dev2.set_mesh(order, meshgen.random_plan(rng, 2, 4, ghosts=True))
-    lib.amr_dev_tape(dev2.h, 0)
