@@ -53,8 +53,10 @@ __global__ __launch_bounds__(NT) void k_exchange_fused(T *__restrict__ pool,
 
 // Variables and threads per workgroup.  Default 8 x 256 (measured best for
 // FP64 and FP32 on the 16^3 workload: 8 x 128 was 3% slower in FP32, 16
-// variables 1-3% slower); MINIAMR_EXCHANGE ("vars,threads" with vars
-// 4/8/16 and threads 128/256) overrides it.
+// variables 1-3% slower; in fused stages 16 variables were 6% slower in
+// FP64 and 2% faster in FP32, 4 variables 11% / 35% slower);
+// MINIAMR_EXCHANGE ("vars,threads" with vars 4/8/16 and threads 128/256)
+// overrides it (fused stages take the variable count, at 256 threads).
 struct ExShape {
    int kv = kVars, nt = 256;
 };
@@ -116,9 +118,20 @@ int launch_fused(amr_dev *d, int dir, int start, int num)
    Lay L;
    if (int rc = amr_lay(d, start, num, &L))
       return rc;
-   dim3 grid(n, (num + kVars - 1) / kVars);
-   hipLaunchKernelGGL((k_exchange_fused<T, kVars, 256>), grid, dim3(256), 0, d->stream,
-                      (T *) d->pool, d->d_fjobs + off, L, start, start + num);
+   const ExShape sh = ex_shape();
+   if (sh.kv == 16) {
+      dim3 grid(n, (num + 15) / 16);
+      hipLaunchKernelGGL((k_exchange_fused<T, 16, 256>), grid, dim3(256), 0, d->stream,
+                         (T *) d->pool, d->d_fjobs + off, L, start, start + num);
+   } else if (sh.kv == 4) {
+      dim3 grid(n, (num + 3) / 4);
+      hipLaunchKernelGGL((k_exchange_fused<T, 4, 256>), grid, dim3(256), 0, d->stream,
+                         (T *) d->pool, d->d_fjobs + off, L, start, start + num);
+   } else {
+      dim3 grid(n, (num + kVars - 1) / kVars);
+      hipLaunchKernelGGL((k_exchange_fused<T, kVars, 256>), grid, dim3(256), 0, d->stream,
+                         (T *) d->pool, d->d_fjobs + off, L, start, start + num);
+   }
    HIP_TRY(d, hipGetLastError());
    return AMR_OK;
 }
