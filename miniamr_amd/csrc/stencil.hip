@@ -142,9 +142,9 @@ __global__ void k_convert_out(const T *__restrict__ pool, double *__restrict__ s
 // ring bookkeeping are shared by several variables.  Measured on the 16^3
 // workload, fused stages, ms per launch for 1 / 2 / 4 variables: FP64
 // 3.240 / 3.093 / 3.170, FP32 2.537 / 1.825 / 1.805, BF16 2.166 / 1.535 /
-// 1.443; so two for FP64 and FP32, four for BF16.  MINIAMR_STENCIL_VPW=
+// 1.443; so two for FP64, four for FP32 and BF16.  MINIAMR_STENCIL_VPW=
 // 1|2|4 overrides the choice.
-template <typename T> struct Vpw { static constexpr int v = sizeof(T) == 2 ? 4 : 2; };
+template <typename T> struct Vpw { static constexpr int v = sizeof(T) == 8 ? 2 : 4; };
 
 template <typename T>
 static int stencil_vpw()
