@@ -450,6 +450,11 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
                      "traffic": traffic,
+                     # PMC-measured bytes per launch over the launch time: the
+                     # fraction of the HBM peak the kernel actually moves
+                     "traffic_frac": (round(traffic / (stencil_s / launches) / 1e9 /
+                                            HBM_PEAK_GBS, 4)
+                                      if traffic and launches and stencil_s > 0 else None),
                      "kernel": "k_stencil_fused + k_stencil (27-point; every stage but "
                                "a timestep's last reads same-level faces in place), %d "
                                "launches, %.3f ms avg" % (
