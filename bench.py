@@ -6,8 +6,12 @@ workload, ~3-5.5k active blocks) on one MI355X.
 A *step* is one timestep of the drop-in driver: 20 stages, each an
 exchange of all 40 variables plus the 27-point stencil over every active
 block (checksums every 5 stages), then object motion and, every 4th
-timestep, refinement.  The workload has 10 timesteps, so the default
---warmup 2 --steps 8 times timesteps 3..10 of the real run.  Cells start
+timestep, refinement.  The workload has 10 timesteps; steps run through
+back-to-back fresh runs of it (step t is timestep t % 10 + 1 of run
+t // 10), each run's setup (allocation, initial values, initial
+refinement) outside the timed region.  The default --warmup 5 --steps 20
+times timesteps 6-10 of the first run, all of the second and 1-5 of the
+third: every timestep of the workload exactly twice.  Cells start
 resident in HBM; nothing crosses PCIe inside the timed region except the
 per-timestep checksum read-back (40 doubles per checksum stage).
 
@@ -18,8 +22,9 @@ roofline: the stencil kernel.  Algorithmic bytes per launch = one read and
         achieved = those bytes / the stencil's device time measured with
         HIP events on the library's stream over the timed region.
 cpu_baseline: the reference's OpenMP build (oracle/_ref/miniAMR_omp.x) on
-        this host, one MPI rank, on a bounded sample of the same workload
-        (its first timestep, --num_tsteps 1), rate from its Summary line.
+        this host, one MPI rank, on the whole 10-timestep workload, rate
+        from its Summary line, next to the GPU's rate over the same
+        timesteps.
 
 --workload adjoint times the differentiation form's reverse pass (the
 adjoint kernels over the whole tape of this workload, FP32 storage) next
@@ -82,6 +87,8 @@ def load_host():
     lib.amr_run_stats_get.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stats)]
     lib.amr_run_report.argtypes = [ctypes.c_void_p]
     lib.amr_run_destroy.argtypes = [ctypes.c_void_p]
+    lib.amr_run_last_error.restype = ctypes.c_char_p
+    lib.amr_run_last_error.argtypes = []
     return lib
 
 
@@ -91,16 +98,16 @@ def stats(lib, run):
     return s
 
 
-def cpu_baseline(timeout=240):
-    """The reference OpenMP build on a bounded sample: the first timestep."""
+def cpu_baseline(per_ts, timeout=300):
+    """The reference OpenMP build on the whole 10-timestep workload, with
+    the GPU's rate over the same timesteps (from the timed steps) beside it."""
     exe = os.path.join(REPO, "oracle", "_ref", "miniAMR_omp.x")
     if not os.path.exists(exe):
         return None
     threads = min(16, os.cpu_count() or 1)
     env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close")
-    args = CFG1.replace("--num_tsteps 10", "--num_tsteps 1").split()
     try:
-        p = subprocess.run([exe] + args, env=env, stdout=subprocess.PIPE,
+        p = subprocess.run([exe] + CFG1.split(), env=env, stdout=subprocess.PIPE,
                            stderr=subprocess.STDOUT, timeout=timeout, cwd="/tmp")
     except (subprocess.TimeoutExpired, OSError) as e:
         print("cpu baseline skipped: %s" % e, file=sys.stderr)
@@ -113,43 +120,87 @@ def cpu_baseline(timeout=240):
         return None
     ts, t, calc, bpts = int(m.group(1)), float(m.group(2)), float(m.group(3)), float(m.group(4))
     updates = bpts * ts * 20 * 16 ** 3 * 40
-    return {"value": updates / t / 1e9, "unit": "GCell-var updates/s", "cores": threads,
-            "kind": "reference",
-            "sample": "reference openmp/ build, 1 MPI rank, %d OpenMP threads: timestep 1 of "
-                      "the same workload (--num_tsteps 1; %.0f blocks/stage, %.3e updates, "
-                      "%.2f s total, %.2f s stencil-only => %.3f GCell-var/s stencil-only)"
-                      % (threads, bpts, updates, t, calc, updates / calc / 1e9)}
+    # the GPU over the same timesteps: every timed step of timestep 1..ts
+    u = sum(per_ts[k][0] for k in range(1, ts + 1) if k in per_ts)
+    s = sum(per_ts[k][1] for k in range(1, ts + 1) if k in per_ts)
+    covered = [k for k in range(1, ts + 1) if k in per_ts]
+    return {"value": round(updates / t / 1e9, 4), "unit": "GCell-var updates/s",
+            "cores": threads, "kind": "reference",
+            "sample": "reference openmp/ build, 1 MPI rank, %d OpenMP threads, the whole "
+                      "workload (%d timesteps, %.0f blocks/stage, %.4e updates): %.2f s "
+                      "(its timer_all: the timesteps plus the initial refinement), %.2f s "
+                      "in the stencil => %.3f GCell-var/s stencil-only"
+                      % (threads, ts, bpts, updates, t, calc, updates / calc / 1e9),
+            "gpu_same_timesteps": {"timesteps": covered,
+                                   "value": round(u / s / 1e9, 4) if s > 0 else None,
+                                   "speedup": round(u / s / (updates / t), 1) if s > 0 else None}}
 
 
 def run_workload(lib, precision, schedule, warmup, steps, sums_path=None):
-    """One full run of the workload; times timesteps warmup+1 .. warmup+steps."""
+    """Timesteps warmup+1 .. warmup+steps of the workload, timed.
+
+    The workload is a 10-timestep run.  Bench steps are its timesteps taken
+    in order from back-to-back fresh runs: step t (0-based over warmup and
+    timed steps) is timestep t % 10 + 1 of run t // 10.  Creating a run
+    (parse, allocate, initial values, initial refinement) and finishing it
+    happen outside the timed region; every timed step is one whole timestep
+    of a real run, and amr_run_step returns after the device is idle.
+    The first run's checksums go to sums_path (all 1,640 of the workload)."""
     os.environ["MINIAMR_PRECISION"] = precision
     os.environ["MINIAMR_SCHEDULE"] = schedule
-    if sums_path:
-        os.environ["AMR_DUMP_SUMS"] = sums_path
-    else:
-        os.environ.pop("AMR_DUMP_SUMS", None)
     argv = [b"miniAMR.x"] + [x.encode() for x in CFG1.split()] + [b"--report_perf", b"0"]
     arr = (ctypes.c_char_p * len(argv))(*argv)
-    t_setup = time.time()
-    run = lib.amr_run_create(len(argv), arr)
-    for _ in range(warmup):
-        lib.amr_run_step(run)
-    s0 = stats(lib, run)
-    setup_s = time.time() - t_setup
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        if lib.amr_run_step(run) == 2:
-            raise SystemExit("conservation check stopped the run")
-    t1 = time.perf_counter()   # amr_run_step returns after the device is idle
-    s1 = stats(lib, run)
-    # finish the remaining timesteps so every checksum of the workload is logged
-    while lib.amr_run_step(run) == 0:
-        pass
-    lib.amr_run_finish(run)
-    lib.amr_run_destroy(run)
+    total = warmup + steps
+    acc = {"updates": 0.0, "block_stages": 0, "stages": 0, "calls": 0,
+           "dev": [0.0] * 4, "elapsed": 0.0}
+    per_ts = {}                     # timestep -> [updates, seconds] over timed steps
+    setup_s, t, first, last = 0.0, 0, True, None
+    while t < total:
+        if sums_path and first:
+            os.environ["AMR_DUMP_SUMS"] = sums_path
+        else:
+            os.environ.pop("AMR_DUMP_SUMS", None)
+        t_setup = time.time()
+        run = lib.amr_run_create(len(argv), arr)
+        if not run:
+            raise M.AmrError("amr_run_create: " + (lib.amr_run_last_error() or b"").decode())
+        setup_s += time.time() - t_setup
+        ts, rc = 0, 0
+        while rc == 0:
+            ts += 1
+            timed = warmup <= t < total
+            s0 = stats(lib, run) if timed else None
+            t0 = time.perf_counter()
+            rc = lib.amr_run_step(run)
+            dt = time.perf_counter() - t0
+            if rc < 0:
+                raise M.AmrError("amr_run_step: " + (lib.amr_run_last_error() or b"").decode())
+            if rc == 2:
+                raise SystemExit("conservation check stopped the run")
+            if timed:
+                s1 = stats(lib, run)
+                du = s1.cell_var_updates - s0.cell_var_updates
+                acc["updates"] += du
+                acc["block_stages"] += s1.block_stages - s0.block_stages
+                acc["stages"] += s1.stages_done - s0.stages_done
+                acc["calls"] += s1.stage_calls - s0.stage_calls
+                for i in range(4):
+                    acc["dev"][i] += s1.device_ms[i] - s0.device_ms[i]
+                acc["elapsed"] += dt
+                p = per_ts.setdefault(ts, [0.0, 0.0])
+                p[0] += du
+                p[1] += dt
+                last = s1
+            t += 1
+            if t >= total and not (sums_path and first):
+                break
+        # a run whose checksums are logged finishes all its timesteps
+        lib.amr_run_finish(run)
+        lib.amr_run_destroy(run)
+        first = False
     os.environ.pop("AMR_DUMP_SUMS", None)
-    return {"s0": s0, "s1": s1, "elapsed": t1 - t0, "setup_s": setup_s}
+    return {"acc": acc, "per_ts": per_ts, "last": last, "setup_s": setup_s,
+            "runs": (total + 9) // 10}
 
 
 BATCH_TABLE = os.path.join(M.PKG, "configs", "batch64.txt")
@@ -355,8 +406,8 @@ def read_sums(path):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=8)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--precision", default=os.environ.get("MINIAMR_PRECISION", "f64"),
                     choices=["f64", "f32", "bf16"])
     ap.add_argument("--schedule", default=os.environ.get("MINIAMR_SCHEDULE", "stage"),
@@ -379,9 +430,8 @@ def main():
     if a.workload == "adjoint":
         main_adjoint(a)
         return
-    if a.warmup + a.steps > 10 or a.steps < 1:
-        print("bench.py: the workload has 10 timesteps; need 1 <= steps and "
-              "warmup + steps <= 10", file=sys.stderr)
+    if a.steps < 1 or a.warmup < 0:
+        print("bench.py: need --steps >= 1 and --warmup >= 0", file=sys.stderr)
         sys.exit(2)
 
     lib = load_host()
@@ -389,12 +439,12 @@ def main():
     tmp = tempfile.mkdtemp(prefix="miniamr_bench_")
     main_sums = os.path.join(tmp, "sums_%s.txt" % a.precision)
     r = run_workload(lib, a.precision, a.schedule, a.warmup, a.steps, main_sums)
-    s0, s1, elapsed = r["s0"], r["s1"], r["elapsed"]
-    updates = s1.cell_var_updates - s0.cell_var_updates
+    acc, s1, elapsed = r["acc"], r["last"], r["acc"]["elapsed"]
+    updates = acc["updates"]
     value = updates / elapsed / 1e9
     ebytes = PREC_BYTES[s1.precision]
-    stencil_s = (s1.device_ms[1] - s0.device_ms[1]) * 1e-3
-    launches = s1.stage_calls - s0.stage_calls
+    stencil_s = acc["dev"][1] * 1e-3
+    launches = acc["calls"]
     achieved = updates * 2 * ebytes / stencil_s / 1e9 if stencil_s > 0 else None
     traffic = None
     pmc = os.path.join(REPO, "profiles", "stencil27_cfg1_%s_pmc.json" % a.precision)
@@ -402,9 +452,13 @@ def main():
         with open(pmc) as f:
             per_bv = json.load(f).get("hbm_bytes_per_block_variable")
         if per_bv and launches:
-            block_vars = (s1.block_stages - s0.block_stages) * s1.num_vars / launches
+            block_vars = acc["block_stages"] * s1.num_vars / launches
             traffic = round(per_bv * block_vars)
-    avg_blocks = (s1.block_stages - s0.block_stages) / max(1, s1.stages_done - s0.stages_done)
+    avg_blocks = acc["block_stages"] / max(1, acc["stages"])
+    timed_ts = sorted(r["per_ts"])
+    same = [r["per_ts"][k] for k in timed_ts]
+    gpu_sample = {"timesteps": timed_ts,
+                  "value": round(sum(u for u, _ in same) / sum(s for _, s in same) / 1e9, 4)}
 
     precision_form = None
     if not a.no_precision_report and a.precision == "f64":
@@ -416,8 +470,7 @@ def main():
             got = read_sums(path)
             n = min(len(ref), len(got))
             errs = [abs(g - f) / abs(f) for g, f in zip(got[:n], ref[:n])]
-            upd = q["s1"].cell_var_updates - q["s0"].cell_var_updates
-            v = upd / q["elapsed"] / 1e9
+            v = q["acc"]["updates"] / q["acc"]["elapsed"] / 1e9
             precision_form[prec] = {
                 "value": round(v, 4), "ratio_vs_f64": round(v / value, 3),
                 "checksums_compared": n,
@@ -438,8 +491,11 @@ def main():
         "dtype": PREC_NAME[s1.precision],
         "data": "synthetic: uniform [0,1) cell values from the reference's generator and seed",
         "config": {"workload": "miniAMR 27-pt, init 4x4x4 blocks of 16^3, 40 vars, "
-                               "num_refine 4, 2 moving objects, timesteps %d-%d of 10"
-                               % (a.warmup + 1, a.warmup + a.steps),
+                               "num_refine 4, 2 moving objects, 10 timesteps x 20 stages",
+                   "steps": "a step is one timestep; steps run through back-to-back fresh "
+                            "runs of the 10-timestep workload (step t = timestep t %% 10 + 1 "
+                            "of run t // 10), run setup untimed: warmup %d + timed %d = "
+                            "%d runs" % (a.warmup, a.steps, r["runs"]),
                    "schedule": a.schedule,
                    "ghosts": ("same-level faces read in place"
                               if os.environ.get("MINIAMR_FUSED", "1") != "0" else "plain"),
@@ -461,10 +517,10 @@ def main():
                                    launches, stencil_s / max(1, launches) * 1e3),
                      "step_level_frac": round(updates * 2 * ebytes / elapsed / 1e9 /
                                               HBM_PEAK_GBS, 4)},
-        "device_ms": {"exchange": round(s1.device_ms[0] - s0.device_ms[0], 2),
-                      "stencil": round(s1.device_ms[1] - s0.device_ms[1], 2),
-                      "checksum": round(s1.device_ms[2] - s0.device_ms[2], 2),
-                      "split_merge": round(s1.device_ms[3] - s0.device_ms[3], 2)},
+        "device_ms": {"exchange": round(acc["dev"][0], 2),
+                      "stencil": round(acc["dev"][1], 2),
+                      "checksum": round(acc["dev"][2], 2),
+                      "split_merge": round(acc["dev"][3], 2)},
         "setup_s": round(r["setup_s"], 2),
         "precision_form": precision_form,
         "batch_form": None if a.no_batch_report else batch_summary(
@@ -472,7 +528,7 @@ def main():
             None if a.no_sequential else run_sequential(batch_runs())),
         "differentiation_form": None if a.no_adjoint_report else adjoint_summary(
             run_adjoint_workload("f32")),
-        "cpu_baseline": None if a.no_cpu_baseline else cpu_baseline(),
+        "cpu_baseline": None if a.no_cpu_baseline else cpu_baseline(r["per_ts"]),
     }
     print(json.dumps(line), flush=True)
 

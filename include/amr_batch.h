@@ -94,12 +94,14 @@ int amr_batch_read_timing(amr_batch *b, double ms[4]);
 /* Driver level. */
 typedef struct amr_batch_run amr_batch_run;
 /* argv of instance i is argvs[i][0 .. argcs[i]) (argv[0] is a program name).
- * Parses and validates every command line (an invalid one ends the process
- * with the reference's message, as amr_run_create does), builds each run's
- * initial mesh, values and refinement, then autotunes if asked. */
+ * Parses and validates every command line, builds each run's initial mesh,
+ * values and refinement, then autotunes if asked.  An invalid command line
+ * or a fatal error prints the reference's message and returns NULL
+ * (amr_run_last_error gives it), as amr_run_create does. */
 amr_batch_run *amr_batch_run_create(int n, const int *argcs, char ***argvs, int autotune);
 /* 0: more timesteps follow; 1: all instances finished; 2: an instance's
- * conservation check stopped it (the others continue). */
+ * conservation check stopped it (the others continue); -1: error (only
+ * amr_batch_run_destroy may follow). */
 int amr_batch_run_step(amr_batch_run *r);
 int amr_batch_run_finish(amr_batch_run *r);
 int amr_batch_run_instances(const amr_batch_run *r);

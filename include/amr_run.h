@@ -16,8 +16,13 @@
  *                    Summary line (ref/profile.c:1298);
  *   amr_run_destroy  free everything.
  *
- * Invalid command lines terminate the process with the reference's message
- * and exit status, as the reference does.
+ * Errors: the reference ends the process on an invalid command line or a
+ * fatal condition (printf + exit(-1), ref/main.c:657-723, ref/block.c:72-75,
+ * ref/comm.c:199-200).  The drop-in executable keeps that; these library
+ * calls print the same message and return instead: amr_run_create returns
+ * NULL, amr_run_step / amr_run_finish return -1, and amr_run_last_error
+ * gives the message (per thread).  After a failed step only
+ * amr_run_destroy may be called.
  */
 #ifndef AMR_RUN_H
 #define AMR_RUN_H
@@ -45,8 +50,10 @@ typedef struct {
 
 amr_run *amr_run_create(int argc, char **argv);
 /* 0: more timesteps follow; 1: last timestep done; 2: stopped by the
- * conservation check (the reference's early return). */
+ * conservation check (the reference's early return); -1: error. */
 int amr_run_step(amr_run *run);
+/* Message of the last failed call on this thread ("" if none). */
+const char *amr_run_last_error(void);
 int amr_run_finish(amr_run *run);
 int amr_run_stats_get(const amr_run *run, amr_run_stats *out);
 void amr_run_report(amr_run *run);

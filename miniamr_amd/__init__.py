@@ -196,6 +196,7 @@ def load_host_library(path=HOST_LIB_PATH):
                                 ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(dp),
                                 ctypes.POINTER(dp)]),
         "amr_run_destroy": (None, [vp]),
+        "amr_run_last_error": (ctypes.c_char_p, []),
         "amr_batch_run_create": (vp, [ip, ctypes.POINTER(ctypes.c_int),
                                       ctypes.POINTER(argv_t), ip]),
         "amr_batch_run_step": (ip, [vp]),
@@ -215,6 +216,10 @@ def load_host_library(path=HOST_LIB_PATH):
     return lib
 
 
+def _host_error(lib):
+    return (lib.amr_run_last_error() or b"").decode(errors="replace").strip()
+
+
 def _argv(args, prog=b"miniAMR.x"):
     toks = [prog] + [x.encode() for x in (args.split() if isinstance(args, str) else args)]
     arr = (ctypes.c_char_p * (len(toks) + 1))(*toks, None)
@@ -228,9 +233,16 @@ class Run:
         self.lib = host or load_host_library()
         self._argc, self._argv = _argv(args)
         self.h = self.lib.amr_run_create(self._argc, self._argv)
+        if not self.h:
+            raise AmrError("amr_run_create: " + _host_error(self.lib))
 
     def step(self):
-        return self.lib.amr_run_step(self.h)
+        """0: more timesteps follow, 1: done, 2: stopped by the conservation
+        check; raises AmrError on a fatal error."""
+        rc = self.lib.amr_run_step(self.h)
+        if rc < 0:
+            raise AmrError("amr_run_step: " + _host_error(self.lib))
+        return rc
 
     def stats(self):
         st = RunStats()
@@ -238,7 +250,10 @@ class Run:
         return st
 
     def finish(self):
-        return self.lib.amr_run_finish(self.h)
+        rc = self.lib.amr_run_finish(self.h)
+        if rc < 0:
+            raise AmrError("amr_run_finish: " + _host_error(self.lib))
+        return rc
 
     def device(self):
         return self.lib.amr_run_device(self.h)
@@ -272,9 +287,14 @@ class BatchRun:
             *[ctypes.cast(k[1], ctypes.POINTER(ctypes.c_char_p)) for k in self._keep])
         self.n = n
         self.h = self.lib.amr_batch_run_create(n, argcs, argvs, 1 if autotune else 0)
+        if not self.h:
+            raise AmrError("amr_batch_run_create: " + _host_error(self.lib))
 
     def step(self):
-        return self.lib.amr_batch_run_step(self.h)
+        rc = self.lib.amr_batch_run_step(self.h)
+        if rc < 0:
+            raise AmrError("amr_batch_run_step: " + _host_error(self.lib))
+        return rc
 
     def stats(self, i):
         st = RunStats()

@@ -14,6 +14,7 @@
 #ifndef AMR_HOST_H
 #define AMR_HOST_H
 
+#include <setjmp.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -152,6 +153,7 @@ typedef struct amr_run {
    amr_vec face_jobs;
    int *scratch_i;             /* max_num_blocks ints */
    double *cellbuf;            /* one slot's cells */
+   int failed;                 /* a library call on this run hit a fatal error */
 } amr_run;
 
 /* params.c */
@@ -170,6 +172,10 @@ double amr_rand01(amr_run *R);
 int amr_build_exchange_plan(amr_run *R);
 int amr_sync_mesh(amr_run *R);
 void amr_fatal(const char *fmt, ...);
+void amr_fatal_quiet(const char *msg);
+int amr_note_error(const char *msg);
+void amr_catch(jmp_buf *jb);
+const char *amr_error_message(void);
 void *amr_xmalloc(size_t bytes);
 void amr_vec_push(amr_vec *v, const void *elem);
 double amr_now(void);

@@ -20,6 +20,7 @@
  * With AMR_DUMP_SUMS / AMR_DUMP_CELLS / AMR_DUMP_BSUMS set, run i writes
  * its checksum log and final dumps to "<path>.<i>".
  */
+#include <setjmp.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -35,6 +36,7 @@ struct amr_batch_run {
    int stopped;          /* some run stopped on the conservation test */
    double tune_ms;       /* stage time chosen by the autotuner, 0 if not run */
    double graph_ms;      /* device time of the timestep graphs so far */
+   int failed;           /* a call hit a fatal error */
 };
 
 static char *suffixed(const char *base, int i)
@@ -47,21 +49,44 @@ static char *suffixed(const char *base, int i)
    return p;
 }
 
+static void release(amr_batch_run *B)
+{
+   for (int i = 0; i < B->n; i++)
+      if (B->runs && B->runs[i]) {
+         amr_free(B->runs[i]);   /* leaves the member context to the batch */
+         free(B->runs[i]);
+      }
+   if (B->batch)
+      amr_batch_destroy(B->batch);
+   free(B->runs);
+   free(B);
+}
+
+/* Fatal errors return here as a NULL batch (amr_fatal, mesh.c). */
 amr_batch_run *amr_batch_run_create(int n, const int *argcs, char ***argvs, int autotune)
 {
+   jmp_buf jb;
+   amr_batch_run *volatile B = NULL;
+   if (setjmp(jb)) {
+      amr_catch(NULL);
+      if (B)
+         release(B);
+      return NULL;
+   }
+   amr_catch(&jb);
    if (n < 1)
       amr_fatal("batch: at least one run is needed\n");
-   amr_batch_run *B = (amr_batch_run *) amr_xmalloc(sizeof(amr_batch_run));
-   B->n = n;
+   B = (amr_batch_run *) amr_xmalloc(sizeof(amr_batch_run));
    B->runs = (amr_run **) amr_xmalloc(sizeof(amr_run *) * n);
+   B->n = n;
    for (int i = 0; i < n; i++) {
       amr_run *R = (amr_run *) amr_xmalloc(sizeof(amr_run));
+      B->runs[i] = R;
       amr_parse_args(&R->P, argcs[i], argvs[i]);
       if (amr_check_input(&R->P))
-         exit(-1);
+         amr_fatal_quiet("invalid input");
       if (!R->P.block_change)
          R->P.block_change = R->P.num_refine;
-      B->runs[i] = R;
    }
    const amr_params *P0 = &B->runs[0]->P;
    for (int i = 1; i < n; i++) {
@@ -97,10 +122,30 @@ amr_batch_run *amr_batch_run_create(int n, const int *argcs, char ***argvs, int 
    amr_batch_set_timing(B->batch, 1);
    double ms[4];
    amr_batch_read_timing(B->batch, ms);
+   amr_catch(NULL);
    return B;
 }
 
+static int batch_step(amr_batch_run *B);
+
+/* -1 after a fatal error; the batch then only accepts destroy. */
 int amr_batch_run_step(amr_batch_run *B)
+{
+   jmp_buf jb;
+   if (!B || B->failed)
+      return -1;
+   if (setjmp(jb)) {
+      amr_catch(NULL);
+      B->failed = 1;
+      return -1;
+   }
+   amr_catch(&jb);
+   int rc = batch_step(B);
+   amr_catch(NULL);
+   return rc;
+}
+
+static int batch_step(amr_batch_run *B)
 {
    int live = 0;
    int *go = (int *) amr_xmalloc(sizeof(int) * B->n);
@@ -150,6 +195,8 @@ int amr_batch_run_step(amr_batch_run *B)
 
 int amr_batch_run_finish(amr_batch_run *B)
 {
+   if (!B || B->failed)
+      return -1;
    for (int i = 0; i < B->n; i++)
       amr_driver_end(B->runs[i]);
    return B->stopped;
@@ -189,20 +236,20 @@ amr_batch *amr_batch_run_batch(amr_batch_run *B)
 
 void amr_batch_run_destroy(amr_batch_run *B)
 {
+   jmp_buf jb;
    if (!B)
       return;
    const char *pc = getenv("AMR_DUMP_CELLS");
    const char *pb = getenv("AMR_DUMP_BSUMS");
-   for (int i = 0; i < B->n; i++) {
-      amr_run *R = B->runs[i];
-      char *c = suffixed(pc, i), *b = suffixed(pb, i);
-      amr_dump_final_to(R, c, b);
-      free(c);
-      free(b);
-      amr_free(R);   /* leaves the member context to the batch */
-      free(R);
+   if (!B->failed && !setjmp(jb)) {
+      amr_catch(&jb);
+      for (int i = 0; i < B->n; i++) {
+         char *c = suffixed(pc, i), *b = suffixed(pb, i);
+         amr_dump_final_to(B->runs[i], c, b);
+         free(c);
+         free(b);
+      }
    }
-   amr_batch_destroy(B->batch);
-   free(B->runs);
-   free(B);
+   amr_catch(NULL);
+   release(B);
 }

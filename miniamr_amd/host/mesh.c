@@ -18,7 +18,9 @@
  * writes, so the plan is simply the set of ghost planes each block
  * receives; it is rebuilt only when the mesh changes.
  */
+#include <setjmp.h>
 #include <stdarg.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include <math.h>
@@ -28,14 +30,56 @@
 #include "amr_adjoint.h"
 #include "amr_batch.h"
 
+/* Fatal errors.  The drop-in executable ends the process the way the
+ * reference does (printf, then exit(-1): ref/block.c:72-75,120-124,
+ * ref/comm.c:199-200).  The library entry points of amr_run.h and
+ * amr_batch.h first arm a recovery point (amr_catch): a fatal error then
+ * records its message, prints it as the reference would, and returns to
+ * that point, so the entry point fails with an error code instead of
+ * ending its caller's process.  State is per thread. */
+static __thread jmp_buf *err_jump;
+static __thread char err_msg[512];
+
+void amr_catch(jmp_buf *jb)
+{
+   err_jump = jb;
+   if (jb)
+      err_msg[0] = 0;
+}
+
+const char *amr_error_message(void)
+{
+   return err_msg;
+}
+
+int amr_note_error(const char *msg)
+{
+   printf("%s", msg);
+   if (!err_msg[0])
+      snprintf(err_msg, sizeof(err_msg), "%s", msg);
+   return 1;
+}
+
+void amr_fatal_quiet(const char *msg)
+{
+   fflush(stdout);
+   if (err_jump) {
+      if (!err_msg[0])
+         snprintf(err_msg, sizeof(err_msg), "%s", msg);
+      longjmp(*err_jump, 1);
+   }
+   exit(-1);
+}
+
 void amr_fatal(const char *fmt, ...)
 {
+   char line[512];
    va_list ap;
    va_start(ap, fmt);
-   vprintf(fmt, ap);
+   vsnprintf(line, sizeof(line), fmt, ap);
    va_end(ap);
-   fflush(stdout);
-   exit(-1);
+   printf("%s", line);
+   amr_fatal_quiet(line);
 }
 
 void *amr_xmalloc(size_t bytes)

@@ -87,12 +87,15 @@ void amr_print_help(void)
       printf("%s\n", lines[i]);
 }
 
+/* Prints the message and the help text, then fails as amr_fatal does. */
 static void die_usage(const char *msg, const char *arg)
 {
+   char line[256] = "";
    if (msg)
-      printf(msg, arg);
+      snprintf(line, sizeof(line), msg, arg);
+   printf("%s", line);
    amr_print_help();
-   exit(-1);
+   amr_fatal_quiet(line[0] ? line : "help requested\n");
 }
 
 /* Reads the integer operand of argv[*i] (advancing *i) or aborts. */
@@ -169,10 +172,8 @@ int amr_parse_args(amr_params *P, int argc, char **argv)
                                             sizeof(amr_object));
          object_num = 0;
       } else if (!strcmp(a, "--object")) {
-         if (object_num >= P->num_objects) {
-            printf("object number greater than num_objects\n");
-            exit(-1);
-         }
+         if (object_num >= P->num_objects)
+            amr_fatal("object number greater than num_objects\n");
          amr_object *o = &P->objects[object_num++];
          o->type = int_arg(argc, argv, &i);
          o->bounce = int_arg(argc, argv, &i);
@@ -190,10 +191,8 @@ int amr_parse_args(amr_params *P, int argc, char **argv)
          die_usage("** Error ** Unknown input parameter %s\n", a);
       }
    }
-   if (object_num != P->num_objects) {
-      printf("Error - number of objects less than specified");
-      exit(-1);
-   }
+   if (object_num != P->num_objects)
+      amr_fatal("Error - number of objects less than specified");
    if (P->reorder == -1)
       P->reorder = P->lb_method ? 0 : 1;
    for (int o = 0; o < P->num_objects; o++)
@@ -211,10 +210,8 @@ int amr_parse_args(amr_params *P, int argc, char **argv)
       P->precision = AMR_PREC_F32;
    else if (prec && !strcmp(prec, "bf16"))
       P->precision = AMR_PREC_BF16;
-   else if (prec && strcmp(prec, "f64")) {
-      printf("precision must be f64, f32 or bf16\n");
-      exit(-1);
-   }
+   else if (prec && strcmp(prec, "f64"))
+      amr_fatal("precision must be f64, f32 or bf16\n");
    const char *sch = getenv("MINIAMR_SCHEDULE");
    P->schedule = (sch && !strcmp(sch, "var")) ? 1 : 0;
    if (P->stencil == 0)   /* coupled variables: keep the stage-major order */
@@ -232,73 +229,57 @@ int amr_check_input(amr_params *P)
    int error = 0;
 
    if (P->init_block_x < 1 || P->init_block_y < 1 || P->init_block_z < 1) {
-      printf("initial blocks on processor must be positive\n");
-      error = 1;
+      error = amr_note_error("initial blocks on processor must be positive\n");
    }
    if (P->max_num_blocks < P->init_block_x * P->init_block_y * P->init_block_z) {
-      printf("max_num_blocks not large enough\n");
-      error = 1;
+      error = amr_note_error("max_num_blocks not large enough\n");
    }
    if (P->nx < 1 || P->ny < 1 || P->nz < 1) {
-      printf("block size must be positive\n");
-      error = 1;
+      error = amr_note_error("block size must be positive\n");
    }
    if ((P->nx / 2) * 2 != P->nx) {
-      printf("block size in x direction must be even\n");
-      error = 1;
+      error = amr_note_error("block size in x direction must be even\n");
    }
    if ((P->ny / 2) * 2 != P->ny) {
-      printf("block size in y direction must be even\n");
-      error = 1;
+      error = amr_note_error("block size in y direction must be even\n");
    }
    if ((P->nz / 2) * 2 != P->nz) {
-      printf("block size in z direction must be even\n");
-      error = 1;
+      error = amr_note_error("block size in z direction must be even\n");
    }
    if (P->num_refine < 0) {
-      printf("number of refinement levels must be non-negative\n");
-      error = 1;
+      error = amr_note_error("number of refinement levels must be non-negative\n");
    }
    if (P->block_change < 0) {
-      printf("number of refinement levels must be non-negative\n");
-      error = 1;
+      error = amr_note_error("number of refinement levels must be non-negative\n");
    }
    if (P->num_vars < 1) {
-      printf("number of variables must be positive\n");
-      error = 1;
+      error = amr_note_error("number of variables must be positive\n");
    }
    if (P->npx * P->npy * P->npz != 1) {
-      printf("number of processors used does not match number allocated\n");
-      error = 1;
+      error = amr_note_error("number of processors used does not match number allocated\n");
    }
    if (P->stencil != 0 && P->stencil != 7 && P->stencil != 27) {
-      printf("illegal value for stencil\n");
-      error = 1;
+      error = amr_note_error("illegal value for stencil\n");
    }
    if (P->stencil == 0 && P->num_vars < 8) {
-      printf("if stencil is 0, num_vars must be more than 8\n");
-      error = 1;
+      error = amr_note_error("if stencil is 0, num_vars must be more than 8\n");
    }
    if (P->stencil == 27 && P->num_refine && !P->uniform_refine)
       printf("WARNING: 27 point stencil with non-uniform refinement: answers may diverge\n");
    if (P->comm_vars == 0 || P->comm_vars > P->num_vars)
       P->comm_vars = P->num_vars;
    if (P->code < 0 || P->code > 2) {
-      printf("code must be 0, 1, or 2\n");
-      error = 1;
+      error = amr_note_error("code must be 0, 1, or 2\n");
    }
    if (P->lb_opt < 0 || P->lb_opt > 2) {
-      printf("lb_opt must be 0, 1, or 2\n");
-      error = 1;
+      error = amr_note_error("lb_opt must be 0, 1, or 2\n");
    }
    /* scope of this build */
    if (!error && P->lb_method >= 2) {
-      printf("this build implements the RCB/Morton initial block order only (not --hilbert)\n");
-      error = 1;
+      error = amr_note_error("this build implements the RCB/Morton initial block order only (not --hilbert)\n");
    }
    if (!error && P->plot_freq) {
-      printf("this build does not write plot files (--plot_freq must be 0)\n");
-      error = 1;
+      error = amr_note_error("this build does not write plot files (--plot_freq must be 0)\n");
    }
    return error;
 }

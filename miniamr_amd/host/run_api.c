@@ -2,33 +2,76 @@
  * run_api.c -- include/amr_run.h: the driver of main.c, one timestep per
  * call, for callers that time or drive the run themselves.
  */
+#include <setjmp.h>
 #include <stdlib.h>
 #include <string.h>
 
 #include "amr.h"
 #include "amr_run.h"
 
+/* Library entry points arm a recovery point for amr_fatal (mesh.c): a
+ * fatal error inside them returns here as an error instead of ending the
+ * caller's process.  A run that failed mid-step keeps failing; only
+ * amr_run_destroy may follow. */
 amr_run *amr_run_create(int argc, char **argv)
 {
-   amr_run *R = (amr_run *) amr_xmalloc(sizeof(amr_run));
+   jmp_buf jb;
+   amr_run *volatile R = NULL;
+   if (setjmp(jb)) {
+      amr_catch(NULL);
+      if (R) {
+         amr_free(R);
+         free(R);
+      }
+      return NULL;
+   }
+   amr_catch(&jb);
+   R = (amr_run *) amr_xmalloc(sizeof(amr_run));
    amr_parse_args(&R->P, argc, argv);
    if (amr_check_input(&R->P))
-      exit(-1);
+      amr_fatal_quiet("invalid input");
    if (!R->P.block_change)
       R->P.block_change = R->P.num_refine;
    amr_allocate(R);
    amr_driver_begin(R);
+   amr_catch(NULL);
    return R;
 }
 
 int amr_run_step(amr_run *R)
 {
-   return amr_driver_step(R);
+   jmp_buf jb;
+   if (!R || R->failed)
+      return -1;
+   if (setjmp(jb)) {
+      amr_catch(NULL);
+      R->failed = 1;
+      return -1;
+   }
+   amr_catch(&jb);
+   int rc = amr_driver_step(R);
+   amr_catch(NULL);
+   return rc;
+}
+
+const char *amr_run_last_error(void)
+{
+   return amr_error_message();
 }
 
 int amr_run_finish(amr_run *R)
 {
+   jmp_buf jb;
+   if (!R || R->failed)
+      return -1;
+   if (setjmp(jb)) {
+      amr_catch(NULL);
+      R->failed = 1;
+      return -1;
+   }
+   amr_catch(&jb);
    amr_driver_end(R);
+   amr_catch(NULL);
    return R->loop.state == 2;
 }
 
@@ -92,9 +135,15 @@ int amr_run_block(const amr_run *R, int in, int32_t *slot, int32_t *level, doubl
 
 void amr_run_destroy(amr_run *R)
 {
+   jmp_buf jb;
    if (!R)
       return;
-   amr_dump_final(R);
+   if (!setjmp(jb)) {
+      amr_catch(&jb);
+      if (!R->failed)
+         amr_dump_final(R);
+   }
+   amr_catch(NULL);
    amr_free(R);
    free(R);
 }

@@ -93,3 +93,23 @@ def test_driver_binary_links_the_hip_library():
     require(M.EXE_PATH, "run __graft_entry__.build()")
     out = subprocess.run(["ldd", M.EXE_PATH], stdout=subprocess.PIPE).stdout.decode()
     assert "libminiamr_dev.so" in out and "oracle" not in out
+
+
+def test_library_calls_return_errors_instead_of_exiting():
+    """An invalid command line and (without a GPU) a failed device
+    creation come back as exceptions with the reference's message; the
+    calling process keeps running."""
+    require(HOST_SO, "run __graft_entry__.build()")
+    host = M.load_host_library(HOST_SO)
+    with pytest.raises(M.AmrError, match="block size in x direction must be even"):
+        M.Run("--nx 5 --num_tsteps 1", host=host)
+    with pytest.raises(M.AmrError, match="Unknown input parameter --bogus"):
+        M.Run("--nx 4 --bogus 3", host=host)
+    with pytest.raises(M.AmrError, match="object number greater than num_objects"):
+        M.Run("--num_objects 0 --object 2 0 0 0 0 0 0 0 0 0 0 0 0 0", host=host)
+    with pytest.raises(M.AmrError, match="must be even"):
+        M.BatchRun(["--nx 4 --num_tsteps 1", "--ny 3 --num_tsteps 1"], host=host)
+    assert host.amr_run_last_error() != b""
+    if not os.path.exists("/dev/kfd"):
+        with pytest.raises(M.AmrError, match="device"):
+            M.Run("--nx 4 --ny 4 --nz 4 --num_tsteps 1", host=host)
