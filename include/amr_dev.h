@@ -179,6 +179,19 @@ int amr_dev_check_sum(amr_dev *dev, int var, double *sum);
 int amr_dev_split(amr_dev *dev, const amr_split_job *jobs, int n);
 int amr_dev_consolidate(amr_dev *dev, const amr_merge_job *jobs, int n);
 
+/* amr_dev_split without device allocation.  No counterpart in the
+ * reference's interface: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  Same jobs, same cells as amr_dev_split; the job list
+ * and the parent-octant snapshots are staged in a static 16 MiB device
+ * array per GPU instead of buffers the context grows.  Limits: at most
+ * 65,536 jobs per call, and one parent octant of all variables
+ * (num_vars * nx*ny*nz/8 cells) must fit 15 MiB (AMR_ERR_ARG otherwise).
+ * Refused while the tape records (AMR_ERR_STATE).  Returns after the work
+ * is done; calls that overlap on one GPU (from several threads, on any of
+ * its contexts) share the array and race. */
+int amr_dev_split_noalloc(amr_dev *dev, const amr_split_job *jobs, int n);
+
 /* Device-side checksum table: n_rows rows of num_vars doubles that
  * amr_dev_stage fills, so a run can enqueue many stages and read the sums
  * back once.  Rows are indexed by the caller (e.g. checksum event number

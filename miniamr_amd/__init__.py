@@ -37,7 +37,7 @@ ABI_SYMBOLS = [
     "amr_dev_read_sums", "amr_dev_stage", "amr_dev_sync", "amr_dev_set_timing",
     "amr_dev_read_timing", "amr_dev_read_exchange_timing", "amr_dev_memory",
     "amr_dev_stage_fused", "amr_dev_set_materials", "amr_dev_stencil_stage",
-    "amr_dev_read_check_counts", "amr_dev_mark_refine",
+    "amr_dev_read_check_counts", "amr_dev_mark_refine", "amr_dev_split_noalloc",
 ]
 
 
@@ -99,6 +99,7 @@ def load_library(path=LIB_PATH):
         "amr_dev_stencil": (ip, [vp, ip]),
         "amr_dev_check_sum": (ip, [vp, ip, dp]),
         "amr_dev_split": (ip, [vp, vp, ip]),
+        "amr_dev_split_noalloc": (ip, [vp, vp, ip]),
         "amr_dev_consolidate": (ip, [vp, vp, ip]),
         "amr_dev_reserve_sums": (ip, [vp, ip]),
         "amr_dev_read_sums": (ip, [vp, ip, ip, dp]),
@@ -409,6 +410,12 @@ class Device:
         j = np.ascontiguousarray(jobs, dtype=SPLIT_JOB)
         self._check(self.lib.amr_dev_split(self.h, j.ctypes.data_as(ctypes.c_void_p), len(j)),
                     "split")
+
+    def split_noalloc(self, jobs):
+        """amr_dev_split staged in the library's static scratch array."""
+        j = np.ascontiguousarray(jobs, dtype=SPLIT_JOB)
+        self._check(self.lib.amr_dev_split_noalloc(self.h, j.ctypes.data_as(ctypes.c_void_p),
+                                                   len(j)), "split_noalloc")
 
     def consolidate(self, jobs):
         j = np.ascontiguousarray(jobs, dtype=MERGE_JOB)

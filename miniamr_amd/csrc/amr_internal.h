@@ -331,6 +331,9 @@ int amr_require_fresh(amr_dev *d, int start, int num, const char *what);
 int amr_launch_block_sums(amr_dev *d, int start, int num);
 int amr_launch_reduce_sums(amr_dev *d, int start, int num, double *dst_row);
 int amr_launch_split(amr_dev *d, const amr_split_job *d_jobs, int n, void *scratch);
+// Looks up this context's device copy of the static scratch array of
+// amr_dev_split_noalloc, once per device (noalloc.hip).
+int amr_noalloc_init(amr_dev *d);
 int amr_launch_merge(amr_dev *d, const amr_merge_job *d_jobs, int n, void *scratch);
 int amr_launch_convert_in(amr_dev *d, int slot);
 int amr_launch_convert_out(amr_dev *d, int slot);

@@ -95,6 +95,8 @@ int amr_dev_create(const amr_dev_config *cfg, amr_dev **out)
    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
       return amr_set_error(d, AMR_ERR_NODEV, "device %d is %s, this library is built for gfx950",
                            cfg->device, prop.gcnArchName);
+   if (int rc = amr_noalloc_init(d))   // static scratch of amr_dev_split_noalloc
+      return rc;
 
    Lay &g = d->g;
    g.nx = cfg->nx; g.ny = cfg->ny; g.nz = cfg->nz;

@@ -550,6 +550,12 @@ int amr_dev_split(amr_dev *d, const amr_split_job *jobs, int n)
    return AMR_OK;
 }
 
+/* The allocation-free variant computes the same cells (include/amr_dev.h). */
+int amr_dev_split_noalloc(amr_dev *d, const amr_split_job *jobs, int n)
+{
+   return amr_dev_split(d, jobs, n);
+}
+
 int amr_dev_consolidate(amr_dev *d, const amr_merge_job *jobs, int n)
 {
    int hx = d->c.nx / 2, hy = d->c.ny / 2, hz = d->c.nz / 2;
