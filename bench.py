@@ -89,6 +89,10 @@ def load_host():
     lib.amr_run_destroy.argtypes = [ctypes.c_void_p]
     lib.amr_run_last_error.restype = ctypes.c_char_p
     lib.amr_run_last_error.argtypes = []
+    lib.amr_run_stages.restype = ctypes.c_int
+    lib.amr_run_stages.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.amr_run_device.restype = ctypes.c_void_p
+    lib.amr_run_device.argtypes = [ctypes.c_void_p]
     return lib
 
 
@@ -201,6 +205,53 @@ def run_workload(lib, precision, schedule, warmup, steps, sums_path=None):
     os.environ.pop("AMR_DUMP_SUMS", None)
     return {"acc": acc, "per_ts": per_ts, "last": last, "setup_s": setup_s,
             "runs": (total + 9) // 10}
+
+
+def lockstep_cell_errors(lib, prec):
+    """The FP64 run and a narrow-storage run of the workload side by side,
+    stage chunk by stage chunk (amr_run_stages), their cells compared on the
+    device (amr_dev_compare_cells) right after every checksum stage: per
+    checksum interval the largest |cell - FP64 cell| over all interior
+    cells of all active blocks and variables.  Untimed."""
+    dev = M.load_library()
+    os.environ["MINIAMR_SCHEDULE"] = "stage"
+    os.environ.pop("AMR_DUMP_SUMS", None)
+    runs = []
+    for p in ("f64", prec):
+        argv = [b"miniAMR.x"] + [x.encode() for x in CFG1.split()] + \
+            [b"--report_perf", b"0", b"--precision", p.encode()]
+        arr = (ctypes.c_char_p * len(argv))(*argv)
+        r = lib.amr_run_create(len(argv), arr)
+        if not r:
+            raise M.AmrError("amr_run_create: " + (lib.amr_run_last_error() or b"").decode())
+        runs.append(r)
+    st = stats(lib, runs[0])
+    nv, spt, cf = st.num_vars, 20, 5
+    ends = [s for s in range(0, spt, cf)] + [spt - 1]      # chunks end after these stages
+    chunks = [ends[0] + 1] + [b - a for a, b in zip(ends, ends[1:])]
+    per_interval, per_var = [], [0.0] * nv
+    out = (ctypes.c_double * nv)()
+    rc = 0
+    while rc == 0:
+        for c, end in zip(chunks, ends):
+            rcs = [lib.amr_run_stages(r, c) for r in runs]
+            if min(rcs) < 0:
+                raise M.AmrError("amr_run_stages: " + (lib.amr_run_last_error() or b"").decode())
+            if end % cf == 0:                                   # a checksum stage
+                if dev.amr_dev_compare_cells(lib.amr_run_device(runs[0]),
+                                             lib.amr_run_device(runs[1]), 0, nv, out):
+                    raise M.AmrError("amr_dev_compare_cells failed")
+                per_interval.append(max(out))
+                per_var = [max(a, b) for a, b in zip(per_var, out)]
+            rc = max(rcs)
+    for r in runs:
+        lib.amr_run_finish(r)
+        lib.amr_run_destroy(r)
+    return {"max_cell_abs_err": float("%.3e" % max(per_interval)),
+            "max_cell_abs_err_per_interval": [float("%.2e" % e) for e in per_interval],
+            "max_cell_abs_err_per_variable": [float("%.2e" % e) for e in per_var],
+            "cells_compared": "every interior cell of every active block and variable, after "
+                              "each of the %d checksum stages" % len(per_interval)}
 
 
 BATCH_TABLE = os.path.join(M.PKG, "configs", "batch64.txt")
@@ -471,11 +522,20 @@ def main():
             n = min(len(ref), len(got))
             errs = [abs(g - f) / abs(f) for g, f in zip(got[:n], ref[:n])]
             v = q["acc"]["updates"] / q["acc"]["elapsed"] / 1e9
+            nv = s1.num_vars
+            ev = [errs[i:i + nv] for i in range(0, n - n % nv, nv)]   # event 0: initial sums
             precision_form[prec] = {
                 "value": round(v, 4), "ratio_vs_f64": round(v / value, 3),
                 "checksums_compared": n,
                 "max_checksum_rel_err_vs_f64": float("%.3e" % max(errs)) if errs else None,
-                "final_event_max_rel_err": float("%.3e" % max(errs[-40:])) if errs else None}
+                "final_event_max_rel_err": float("%.3e" % max(errs[-nv:])) if errs else None,
+                # per checksum interval (every 5th stage, all variables) and
+                # per variable (over all intervals)
+                "checksum_rel_err_per_interval": [float("%.2e" % max(e)) for e in ev[1:]],
+                "checksum_rel_err_per_variable": [
+                    float("%.2e" % max(e[v] for e in ev[1:])) for v in range(nv)]
+                if len(ev) > 1 else None}
+            precision_form[prec].update(lockstep_cell_errors(lib, prec))
 
     line = {
         "metric": "GCell-var updates/s",

@@ -28,6 +28,8 @@
  *   consolidate_blocks() data loop ref/block.c:411-431  amr_dev_consolidate
  *   reset_all + check_objects + refine_level     amr_dev_mark_refine
  *                       ref/refine.c:66-82, 325-527, ref/move.c:65-623
+ *   pack_face / unpack_face (off-rank faces)   amr_dev_pack_faces /
+ *                       ref/comm.c:245-1461          amr_dev_unpack_faces
  *
  * Conventions
  *   - Cell arrays use the reference's index order array[var][i][j][k]
@@ -191,6 +193,51 @@ int amr_dev_consolidate(amr_dev *dev, const amr_merge_job *jobs, int n);
  * is done; calls that overlap on one GPU (from several threads, on any of
  * its contexts) share the array and race. */
 int amr_dev_split_noalloc(amr_dev *dev, const amr_split_job *jobs, int n);
+
+/* Halo message codec: the one-GPU part of an exchange across GPUs.  The
+ * reference packs every off-rank face into an MPI message and unpacks what
+ * it receives (pack_face ref/comm.c:245-990, unpack_face :993-1461,
+ * called from comm() :87-94, :147-151; message lengths ref/init.c:77-117);
+ * across GPUs the packed buffers would travel by RCCL send/recv.  A
+ * message is one face of one block for variables [start, start+num_comm),
+ * variable after variable, in the reference's formats for --code 0, 1 or 2
+ * (code 0: face interiors, restriction as 2x2 sums on the fine sender,
+ * prolongation as value / 4 on the coarse sender replicated by the fine
+ * receiver; codes 1 and 2 add the ghost ring, code 2 prolongs on the send
+ * side).  face_case is the reference's comm_face_case: 0 whole face, 1
+ * whole face with the 27-point ring, 2-5 whole face <-> quarter (this block
+ * is the fine one; quarter = face_case - 2), 6-9 quarter <-> whole face
+ * (this block is the coarse one), +10 for the E/N/U face of the direction.
+ * pack reads the block's boundary plane, unpack writes its ghost plane.
+ * Values are in the context's storage type (double at AMR_PREC_F64, as the
+ * reference's MPI_DOUBLE messages).  buf is device memory holding buf_elems
+ * values; message t occupies [offset, offset + num_comm * len) with len =
+ * amr_dev_face_msg_len(dev, code, dir, face_case, unpack).  Messages of one
+ * pack call may share a block; those of one unpack call must write
+ * different ghost cells.  Enqueue only (the message list is copied first). */
+typedef struct {
+   int32_t slot;       /* block slot */
+   int32_t face_case;  /* 0..19, as above */
+   int64_t offset;     /* first value of the message in buf */
+} amr_face_msg;
+
+/* Values per variable of one message (ref/init.c:77-117): the sender's
+ * length (unpack = 0) or the receiver's (unpack = 1) for this face case;
+ * negative AMR_ERR_* on bad arguments. */
+int amr_dev_face_msg_len(amr_dev *dev, int code, int dir, int face_case, int unpack);
+int amr_dev_pack_faces(amr_dev *dev, int code, int dir, int start, int num_comm,
+                       const amr_face_msg *msgs, int n, void *buf, size_t buf_elems);
+int amr_dev_unpack_faces(amr_dev *dev, int code, int dir, int start, int num_comm,
+                         const amr_face_msg *msgs, int n, const void *buf, size_t buf_elems);
+
+/* Largest |a - b| over the interior cells of the active blocks, for each
+ * variable of [start, start+num): two contexts on one device holding the
+ * same mesh (block size, variables, slots and active list), in any
+ * storage precisions -- the cell error of a narrow storage form against an
+ * FP64 run after each checksum interval.  No counterpart in the reference
+ * (its cells are FP64 only).  Waits for b's enqueued work; returns after
+ * the comparison. */
+int amr_dev_compare_cells(amr_dev *a, amr_dev *b, int start, int num, double *max_abs);
 
 /* Device-side checksum table: n_rows rows of num_vars doubles that
  * amr_dev_stage fills, so a run can enqueue many stages and read the sums

@@ -52,6 +52,15 @@ amr_run *amr_run_create(int argc, char **argv);
 /* 0: more timesteps follow; 1: last timestep done; 2: stopped by the
  * conservation check (the reference's early return); -1: error. */
 int amr_run_step(amr_run *run);
+/* The next n stages of the current timestep (stage-major order), finished
+ * on the device when the call returns, so the caller can look at the
+ * cells between stages -- e.g. amr_dev_compare_cells against a run of the
+ * same mesh in another precision after every checksum interval.  The call
+ * that reaches the timestep's last stage also finishes the timestep
+ * (checksums, conservation test, motion, refinement) and returns what
+ * amr_run_step would; the others return 0.  Do not mix with amr_run_step
+ * inside a timestep. */
+int amr_run_stages(amr_run *run, int n);
 /* Message of the last failed call on this thread ("" if none). */
 const char *amr_run_last_error(void);
 int amr_run_finish(amr_run *run);

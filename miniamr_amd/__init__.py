@@ -38,6 +38,8 @@ ABI_SYMBOLS = [
     "amr_dev_read_timing", "amr_dev_read_exchange_timing", "amr_dev_memory",
     "amr_dev_stage_fused", "amr_dev_set_materials", "amr_dev_stencil_stage",
     "amr_dev_read_check_counts", "amr_dev_mark_refine", "amr_dev_split_noalloc",
+    "amr_dev_face_msg_len", "amr_dev_pack_faces", "amr_dev_unpack_faces",
+    "amr_dev_compare_cells",
 ]
 
 
@@ -52,6 +54,7 @@ FACE_JOB = np.dtype([("kind", "<i4"), ("face", "<i4"), ("dst", "<i4"), ("src", "
                      ("iq", "<i4"), ("jq", "<i4")])
 SPLIT_JOB = np.dtype([("parent", "<i4"), ("child", "<i4"), ("octant", "<i4"), ("pad", "<i4")])
 MERGE_JOB = np.dtype([("dst", "<i4"), ("child", "<i4"), ("octant", "<i4"), ("pad", "<i4")])
+FACE_MSG = np.dtype([("slot", "<i4"), ("face_case", "<i4"), ("offset", "<i8")])
 
 
 class BatchShape(ctypes.Structure):
@@ -100,6 +103,10 @@ def load_library(path=LIB_PATH):
         "amr_dev_check_sum": (ip, [vp, ip, dp]),
         "amr_dev_split": (ip, [vp, vp, ip]),
         "amr_dev_split_noalloc": (ip, [vp, vp, ip]),
+        "amr_dev_face_msg_len": (ip, [vp, ip, ip, ip, ip]),
+        "amr_dev_compare_cells": (ip, [vp, vp, ip, ip, dp]),
+        "amr_dev_pack_faces": (ip, [vp, ip, ip, ip, ip, vp, ip, vp, ctypes.c_size_t]),
+        "amr_dev_unpack_faces": (ip, [vp, ip, ip, ip, ip, vp, ip, vp, ctypes.c_size_t]),
         "amr_dev_consolidate": (ip, [vp, vp, ip]),
         "amr_dev_reserve_sums": (ip, [vp, ip]),
         "amr_dev_read_sums": (ip, [vp, ip, ip, dp]),
@@ -188,6 +195,7 @@ def load_host_library(path=HOST_LIB_PATH):
     sig = {
         "amr_run_create": (vp, [ip, argv_t]),
         "amr_run_step": (ip, [vp]),
+        "amr_run_stages": (ip, [vp, ip]),
         "amr_run_finish": (ip, [vp]),
         "amr_run_stats_get": (ip, [vp, ctypes.POINTER(RunStats)]),
         "amr_run_report": (None, [vp]),
@@ -416,6 +424,30 @@ class Device:
         j = np.ascontiguousarray(jobs, dtype=SPLIT_JOB)
         self._check(self.lib.amr_dev_split_noalloc(self.h, j.ctypes.data_as(ctypes.c_void_p),
                                                    len(j)), "split_noalloc")
+
+    def face_msg_len(self, code, dir, face_case, unpack=False):
+        """Values per variable of one message (ref/init.c:77-117)."""
+        n = self.lib.amr_dev_face_msg_len(self.h, code, dir, face_case, int(unpack))
+        self._check(min(n, 0), "face_msg_len")
+        return n
+
+    def _codec(self, fn, what, code, dir, start, num, msgs, buf):
+        """buf: a torch tensor on the device (its storage is the message
+        buffer) or an integer device address with a length: (addr, n)."""
+        m = np.ascontiguousarray(msgs, dtype=FACE_MSG)
+        if isinstance(buf, tuple):
+            addr, n = buf
+        else:
+            addr, n = buf.data_ptr(), buf.numel()
+        self._check(fn(self.h, code, dir, start, num, m.ctypes.data_as(ctypes.c_void_p), len(m),
+                       ctypes.c_void_p(addr), n), what)
+
+    def pack_faces(self, code, dir, start, num, msgs, buf):
+        self._codec(self.lib.amr_dev_pack_faces, "pack_faces", code, dir, start, num, msgs, buf)
+
+    def unpack_faces(self, code, dir, start, num, msgs, buf):
+        self._codec(self.lib.amr_dev_unpack_faces, "unpack_faces", code, dir, start, num, msgs,
+                    buf)
 
     def consolidate(self, jobs):
         j = np.ascontiguousarray(jobs, dtype=MERGE_JOB)

@@ -104,6 +104,17 @@ struct amr_dev {
    size_t scratch_bytes = 0;
    void *d_jobbuf = nullptr;        // split/merge job array
    size_t jobbuf_bytes = 0;
+   // halo codec (codec.hip): message formats of every code, direction and
+   // face case (built on first use), values per variable of each, and the
+   // message list of the current call
+   void *d_codec = nullptr;
+   int codec_len[3 * 3 * 20 * 2] = {};
+   void *d_msgs = nullptr;
+   size_t msgs_bytes = 0;
+   // amr_dev_compare_cells (compare.hip): host copy of the active slots and
+   // the per-variable maxima
+   std::vector<int32_t> h_sorted;
+   unsigned long long *d_cmp = nullptr;
 
    // Two-interior records (unless MINIAMR_FUSED=0 at creation): a second copy of the
    // I and ZI parts, so a stage can read one and write the other.  par[v]

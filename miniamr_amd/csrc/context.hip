@@ -172,6 +172,9 @@ int amr_dev_destroy(amr_dev *d)
    hipFree(d->d_stage);
    hipFree(d->d_scratch);
    hipFree(d->d_jobbuf);
+   hipFree(d->d_codec);
+   hipFree(d->d_msgs);
+   hipFree(d->d_cmp);
    hipFree(d->d_fjobs);
    hipFree(d->d_redir);
    hipFree(d->d_a0);
@@ -369,6 +372,7 @@ int amr_set_mesh_impl(amr_dev *d, const int32_t *sorted, int n_active,
    if (n_active)
       HIP_TRY(d, hipMemcpy(d->d_sorted, sorted, sizeof(int32_t) * n_active,
                            hipMemcpyHostToDevice));
+   d->h_sorted.assign(sorted, sorted + n_active);
    if (nj > d->jobs_cap) {
       hipFree(d->d_jobs);
       d->d_jobs = nullptr;

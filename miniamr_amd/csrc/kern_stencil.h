@@ -14,6 +14,19 @@ constexpr int kCols = 4;       // columns per thread: ny*nz <= 1024
 constexpr int kPer = 8;        // plane elements per thread: gy*gz <= 2048
 constexpr int kDepth = 4;      // planes of loads in flight per workgroup
 
+// Workgroups are dispatched round-robin over the device's 8 XCDs in
+// flattened-id order, and each XCD has its own L2.  xcd_remap(id, total)
+// turns the flattened id into a work index such that each XCD runs one
+// contiguous share of the work (XCD x gets indices [x*per, (x+1)*per),
+// with the remainder spread over the first XCDs): consecutive items --
+// neighbouring blocks in sorted order -- then run on the same XCD and the
+// face planes one reads from the other's record are found in that L2.
+__device__ __forceinline__ int xcd_remap(int id, int total)
+{
+   const int x = id & 7, p = id >> 3, per = total >> 3, rem = total & 7;
+   return x < rem ? x * (per + 1) + p : rem * (per + 1) + (x - rem) * per + p;
+}
+
 // Synchronisation and reduction over the NT threads that share one work
 // item: the whole workgroup (NT == kThreads) or one wavefront (NT == 64).
 // A wavefront's LDS operations complete in issue order, so within a wave

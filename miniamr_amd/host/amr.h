@@ -84,6 +84,7 @@ typedef struct { void *v; int n, cap, elem; } amr_vec;
 /* State of the timestep loop between amr_driver_step calls. */
 typedef struct {
    int ts, comm_stage, state, n_rows;
+   int stage_pos;              /* stages of timestep ts done by amr_driver_stages */
    double delta, sim_time, t_start;
    double dev_ms[4];
    int64_t stage_calls;
@@ -196,13 +197,15 @@ void amr_consolidate_blocks(amr_run *R);
 int amr_driver(amr_run *R);
 int amr_driver_begin(amr_run *R);
 int amr_driver_step(amr_run *R);
+int amr_driver_stages(amr_run *R, int n);
 int amr_driver_step_begin(amr_run *R);
 void amr_driver_step_enqueue(amr_run *R);
 void amr_driver_step_advance(amr_run *R);
 int amr_driver_step_end(amr_run *R);
 void amr_driver_end(amr_run *R);
 void amr_driver_set_sum_log(amr_run *R, const char *path);
-int amr_dump_final_to(amr_run *R, const char *cells_path, const char *bsums_path);
+int amr_dump_final_to(amr_run *R, const char *cells_path, const char *bsums_path,
+                      const char *sample_path);
 double amr_calc_time_step(amr_run *R);
 void amr_report(amr_run *R);                 /* report.c */
 void amr_count_comm(amr_run *R, int calls);  /* driver.c */

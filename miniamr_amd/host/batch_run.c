@@ -17,7 +17,7 @@
  * sequence; everything else (block size, variables, refinement levels,
  * objects, seed, timesteps) may differ.
  *
- * With AMR_DUMP_SUMS / AMR_DUMP_CELLS / AMR_DUMP_BSUMS set, run i writes
+ * With AMR_DUMP_SUMS / AMR_DUMP_CELLS / AMR_DUMP_BSUMS / AMR_DUMP_SAMPLE set, run i writes
  * its checksum log and final dumps to "<path>.<i>".
  */
 #include <setjmp.h>
@@ -241,13 +241,15 @@ void amr_batch_run_destroy(amr_batch_run *B)
       return;
    const char *pc = getenv("AMR_DUMP_CELLS");
    const char *pb = getenv("AMR_DUMP_BSUMS");
+   const char *ps = getenv("AMR_DUMP_SAMPLE");
    if (!B->failed && !setjmp(jb)) {
       amr_catch(&jb);
       for (int i = 0; i < B->n; i++) {
-         char *c = suffixed(pc, i), *b = suffixed(pb, i);
-         amr_dump_final_to(B->runs[i], c, b);
+         char *c = suffixed(pc, i), *b = suffixed(pb, i), *q = suffixed(ps, i);
+         amr_dump_final_to(B->runs[i], c, b, q);
          free(c);
          free(b);
+         free(q);
       }
    }
    amr_catch(NULL);

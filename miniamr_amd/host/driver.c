@@ -143,24 +143,33 @@ static void count_flops0(amr_run *R, int start, int number, int calc_stage)
    }
 }
 
-/* Enqueue one timestep; checksum event e of the timestep goes to row e. */
+/* Stages s0 .. s1-1 of a timestep in stage-major order; checksum event e
+ * of the timestep goes to row e. */
+static void enqueue_stage_range(amr_run *R, int comm_stage0, int s0, int s1)
+{
+   const amr_params *P = &R->P;
+   int V = P->num_vars;
+   for (int stage = s0; stage < s1; stage++) {
+      int row = (P->checksum_freq && !(stage % P->checksum_freq))
+                   ? stage / P->checksum_freq : -1;
+      for (int start = 0; start < V; start += P->comm_vars) {
+         int number = (start + P->comm_vars > V) ? V - start : P->comm_vars;
+         enqueue_stage(R, start, number, stage, comm_stage0 + stage, row);
+         if (P->stencil == 0)
+            count_flops0(R, start, number, comm_stage0 + stage);
+         R->loop.stage_calls++;
+         amr_count_comm(R, 1);
+      }
+   }
+}
+
+/* Enqueue one timestep. */
 static void enqueue_timestep(amr_run *R, int comm_stage0)
 {
    const amr_params *P = &R->P;
    int V = P->num_vars;
    if (P->schedule == 0) {
-      for (int stage = 0; stage < P->stages_per_ts; stage++) {
-         int row = (P->checksum_freq && !(stage % P->checksum_freq))
-                      ? stage / P->checksum_freq : -1;
-         for (int start = 0; start < V; start += P->comm_vars) {
-            int number = (start + P->comm_vars > V) ? V - start : P->comm_vars;
-            enqueue_stage(R, start, number, stage, comm_stage0 + stage, row);
-            if (P->stencil == 0)
-               count_flops0(R, start, number, comm_stage0 + stage);
-            R->loop.stage_calls++;
-            amr_count_comm(R, 1);
-         }
-      }
+      enqueue_stage_range(R, comm_stage0, 0, P->stages_per_ts);
    } else {
       /* Variables per group: as many as keep the group's active slabs
        * within ~128 MiB, so they stay in the 256 MiB last-level cache
@@ -363,6 +372,33 @@ int amr_driver_step(amr_run *R)
    return amr_driver_step_end(R);
 }
 
+/* The next n stages of the current timestep, stage-major, completed on
+ * the device before returning; the call that reaches the timestep's last
+ * stage finishes the timestep as amr_driver_step does and returns its
+ * code, the others return 0. */
+int amr_driver_stages(amr_run *R, int n)
+{
+   amr_loop *L = &R->loop;
+   const int spt = R->P.stages_per_ts;
+   if (L->stage_pos == 0) {
+      int st = amr_driver_step_begin(R);
+      if (st)
+         return st;
+   }
+   int s1 = L->stage_pos + (n > 0 ? n : 1);
+   if (s1 > spt)
+      s1 = spt;
+   enqueue_stage_range(R, L->comm_stage, L->stage_pos, s1);
+   L->stage_pos = s1;
+   if (s1 < spt) {
+      amr_dev_check(R, amr_dev_sync(R->dev), "sync");
+      return 0;
+   }
+   L->stage_pos = 0;
+   amr_driver_step_advance(R);
+   return amr_driver_step_end(R);
+}
+
 /* ref/driver.c:162-164 and the timer bookkeeping for the report. */
 void amr_driver_end(amr_run *R)
 {
@@ -405,13 +441,31 @@ int amr_driver(amr_run *R)
 /* Final-state dumps in the formats of the golden-vector harness. */
 int amr_dump_final(amr_run *R)
 {
-   return amr_dump_final_to(R, getenv("AMR_DUMP_CELLS"), getenv("AMR_DUMP_BSUMS"));
+   return amr_dump_final_to(R, getenv("AMR_DUMP_CELLS"), getenv("AMR_DUMP_BSUMS"),
+                            getenv("AMR_DUMP_SAMPLE"));
 }
 
-int amr_dump_final_to(amr_run *R, const char *pc, const char *pb)
+/* The cell sample of AMR_DUMP_SAMPLE: AMR_SAMPLE_CELLS cells, each drawn as
+ * (sorted-list index, variable, i, j, k) with ghosts included, from a
+ * splitmix64 sequence seeded AMR_SAMPLE_SEED -- the same draws as the
+ * golden-vector harness of the reference build makes. */
+#define AMR_SAMPLE_CELLS 16384
+#define AMR_SAMPLE_SEED 0x5eed5eed5eedULL
+
+static uint64_t splitmix64(uint64_t *x)
+{
+   uint64_t z = (*x += 0x9e3779b97f4a7c15ULL);
+   z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+   z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+   return z ^ (z >> 31);
+}
+
+typedef struct { int32_t in, v, i, j, k, pad; double val; } sample_rec;
+
+int amr_dump_final_to(amr_run *R, const char *pc, const char *pb, const char *ps)
 {
    const amr_params *P = &R->P;
-   if (!pc && !pb)
+   if (!pc && !pb && !ps)
       return 0;
    int gx = P->nx + 2, gy = P->ny + 2, gz = P->nz + 2;
    size_t slab = (size_t) gx * gy * gz;
@@ -424,6 +478,18 @@ int amr_dump_final_to(amr_run *R, const char *pc, const char *pb)
    hdr[0] = 0x414d5332;
    if (fb)
       fwrite(hdr, sizeof(hdr), 1, fb);
+   sample_rec *smp = NULL;
+   if (ps && na > 0) {
+      uint64_t x = AMR_SAMPLE_SEED;
+      smp = (sample_rec *) amr_xmalloc(sizeof(sample_rec) * AMR_SAMPLE_CELLS);
+      for (int s = 0; s < AMR_SAMPLE_CELLS; s++) {
+         smp[s].in = (int32_t) (splitmix64(&x) % (uint64_t) na);
+         smp[s].v = (int32_t) (splitmix64(&x) % (uint64_t) P->num_vars);
+         smp[s].i = (int32_t) (splitmix64(&x) % (uint64_t) gx);
+         smp[s].j = (int32_t) (splitmix64(&x) % (uint64_t) gy);
+         smp[s].k = (int32_t) (splitmix64(&x) % (uint64_t) gz);
+      }
+   }
    for (int in = 0; in < na; in++) {
       int slot = R->sorted_slot[in];
       const amr_block *b = &R->blocks[slot];
@@ -451,8 +517,25 @@ int amr_dump_final_to(amr_run *R, const char *pc, const char *pb)
             fwrite(s, sizeof(s), 1, fb);
          }
       }
+      if (smp)
+         for (int s = 0; s < AMR_SAMPLE_CELLS; s++)
+            if (smp[s].in == in)
+               smp[s].val = R->cellbuf[slab * smp[s].v + ((size_t) smp[s].i * gy + smp[s].j) * gz
+                                       + smp[s].k];
    }
    if (fc) fclose(fc);
    if (fb) fclose(fb);
+   if (ps) {
+      FILE *f = fopen(ps, "wb");
+      if (f) {
+         hdr[0] = 0x414d5333;
+         hdr[7] = smp ? AMR_SAMPLE_CELLS : 0;
+         fwrite(hdr, sizeof(hdr), 1, f);
+         if (smp)
+            fwrite(smp, sizeof(sample_rec), AMR_SAMPLE_CELLS, f);
+         fclose(f);
+      }
+   }
+   free(smp);
    return 0;
 }

@@ -54,6 +54,22 @@ int amr_run_step(amr_run *R)
    return rc;
 }
 
+int amr_run_stages(amr_run *R, int n)
+{
+   jmp_buf jb;
+   if (!R || R->failed)
+      return -1;
+   if (setjmp(jb)) {
+      amr_catch(NULL);
+      R->failed = 1;
+      return -1;
+   }
+   amr_catch(&jb);
+   int rc = amr_driver_stages(R, n);
+   amr_catch(NULL);
+   return rc;
+}
+
 const char *amr_run_last_error(void)
 {
    return amr_error_message();

@@ -787,3 +787,227 @@ int amr_dev_tape(amr_dev *d, int on)
       return fail(d, AMR_ERR_STATE, "the CPU oracle does not record a tape");
    return AMR_OK;
 }
+
+/* ---- halo message codec: pack_face / unpack_face (ref/comm.c:245-1461),
+ * lengths ref/init.c:77-117.  Each direction's face plane is addressed as
+ * P[a][b] with (a, b) the reference's loop axes: (j, k) for x, (i, k) for
+ * y, (i, j) for z; `p` is the plane's index along the direction.  The
+ * reference's three per-direction copies of every loop are one loop here
+ * over (a, b).  Buffers are host memory in this CPU library. ---- */
+
+static double *plane_cell(amr_dev *d, int m, int slot, int dir, int p, int a, int b)
+{
+   int i = dir == 0 ? p : a, j = dir == 0 ? a : dir == 1 ? p : b, k = dir == 2 ? p : b;
+   return &A(d, m, slot, i, j, k);
+}
+
+static void plane_ext(const amr_dev *d, int dir, int *na, int *nb, int *nd)
+{
+   *na = dir == 0 ? d->c.ny : d->c.nx;
+   *nb = dir == 2 ? d->c.ny : d->c.nz;
+   *nd = dir == 0 ? d->c.nx : dir == 1 ? d->c.ny : d->c.nz;
+}
+
+/* One message; returns the number of values written (per call), or -1. */
+static long pack_one(amr_dev *d, int code, int dir, int fc, int slot, int start, int num,
+                     double *out)
+{
+   int na, nb, nd;
+   plane_ext(d, dir, &na, &nb, &nd);
+   const int c = fc % 10, p = fc >= 10 ? nd : 1, ha = na / 2, hb = nb / 2;
+   const int lo_a = c % 2 == 0, lo_b = (c / 2) % 2 == 1;
+   long n = 0;
+   for (int m = start; m < start + num; m++) {
+#define P(a, b) (*plane_cell(d, m, slot, dir, p, (a), (b)))
+      if (code == 0 && c < 2) {                 /* ref/comm.c:264-269, 307-321, 353-367 */
+         int ra = c == 1 && dir > 0, rb = c == 1 && dir == 2;
+         for (int a = ra ? 0 : 1; a <= (ra ? na + 1 : na); a++)
+            for (int b = rb ? 0 : 1; b <= (rb ? nb + 1 : nb); b++)
+               out[n++] = P(a, b);
+      } else if (code == 0 && c <= 5) {         /* :270-278 and twins */
+         for (int a = 1; a < na; a += 2)
+            for (int b = 1; b < nb; b += 2)
+               out[n++] = P(a, b) + P(a, b + 1) + P(a + 1, b) + P(a + 1, b + 1);
+      } else if (code == 0) {                   /* :279-299 and twins */
+         for (int a = lo_a ? 1 : ha + 1; a <= (lo_a ? ha : na); a++)
+            for (int b = lo_b ? 1 : hb + 1; b <= (lo_b ? hb : nb); b++)
+               out[n++] = P(a, b) / 4.0;
+      } else if (c < 2) {                       /* :413-418, 665-670 */
+         for (int a = 0; a <= na + 1; a++)
+            for (int b = 0; b <= nb + 1; b++)
+               out[n++] = P(a, b);
+      } else if (c <= 5) {                      /* :419-465, 670-718 */
+         /* rows: the ring row 0 first (even cases), the pairs, the ring
+          * row na+1 last (odd cases); in each row the ring column 0 first
+          * (cases 2, 3) or nb+1 last (4, 5); pairs sum as [a][b] + [a][b+1]
+          * + [a+1][b] + [a+1][b+1] */
+         int ga[64][2], gb[64][2], nga = 0, ngb = 0;
+         if (lo_a) { ga[nga][0] = ga[nga][1] = 0; nga++; }
+         for (int a = 1; a < na; a += 2, nga++) { ga[nga][0] = a; ga[nga][1] = a + 1; }
+         if (!lo_a) { ga[nga][0] = ga[nga][1] = na + 1; nga++; }
+         if (lo_b) { gb[ngb][0] = gb[ngb][1] = 0; ngb++; }
+         for (int b = 1; b < nb; b += 2, ngb++) { gb[ngb][0] = b; gb[ngb][1] = b + 1; }
+         if (!lo_b) { gb[ngb][0] = gb[ngb][1] = nb + 1; ngb++; }
+         for (int x = 0; x < nga; x++)
+            for (int y = 0; y < ngb; y++) {
+               double v = 0.0;
+               int first = 1;
+               for (int a = ga[x][0]; a <= ga[x][1]; a++)
+                  for (int b = gb[y][0]; b <= gb[y][1]; b++) {
+                     v = first ? P(a, b) : v + P(a, b);
+                     first = 0;
+                  }
+               out[n++] = v;
+            }
+      } else if (code == 1) {                   /* :466-486 and twins */
+         for (int a = lo_a ? 0 : ha; a <= (lo_a ? ha + 1 : na + 1); a++)
+            for (int b = lo_b ? 0 : hb; b <= (lo_b ? hb + 1 : nb + 1); b++)
+               out[n++] = P(a, b) / 4.0;
+      } else {                                  /* code 2: :719-763 and twins */
+         int a0 = lo_a ? 1 : ha + 1, a1 = lo_a ? ha : na;
+         int b0 = lo_b ? 1 : hb + 1, b1 = lo_b ? hb : nb;
+         for (int r = a0 - 1; r <= a1 + 1; r++)
+            for (int rep = 0; rep < ((r >= a0 && r <= a1) ? 2 : 1); rep++) {
+               out[n++] = P(r, b0 - 1) / 4.0;
+               for (int b = b0; b <= b1; b++) {
+                  out[n] = out[n + 1] = P(r, b) / 4.0;
+                  n += 2;
+               }
+               out[n++] = P(r, b1 + 1) / 4.0;
+            }
+      }
+   }
+   return n;
+}
+
+static long unpack_one(amr_dev *d, int code, int dir, int fc, int slot, int start, int num,
+                       const double *in)
+{
+   int na, nb, nd;
+   plane_ext(d, dir, &na, &nb, &nd);
+   const int c = fc % 10, p = fc >= 10 ? nd + 1 : 0, ha = na / 2, hb = nb / 2;
+   const int lo_a = c % 2 == 0, lo_b = (c / 2) % 2 == 1;
+   long n = 0;
+   for (int m = start; m < start + num; m++) {
+      if (code == 0 && c < 2) {                 /* ref/comm.c:1015-1019 and twins */
+         int ra = c == 1 && dir > 0, rb = c == 1 && dir == 2;
+         for (int a = ra ? 0 : 1; a <= (ra ? na + 1 : na); a++)
+            for (int b = rb ? 0 : 1; b <= (rb ? nb + 1 : nb); b++)
+               P(a, b) = in[n++];
+      } else if (code == 0 && c <= 5) {         /* :1020-1028 and twins */
+         for (int a = 1; a < na; a += 2)
+            for (int b = 1; b < nb; b += 2, n++)
+               P(a, b) = P(a, b + 1) = P(a + 1, b) = P(a + 1, b + 1) = in[n];
+      } else if (code == 0) {                   /* :1029-1048 and twins */
+         for (int a = lo_a ? 1 : ha + 1; a <= (lo_a ? ha : na); a++)
+            for (int b = lo_b ? 1 : hb + 1; b <= (lo_b ? hb : nb); b++)
+               P(a, b) = in[n++];
+      } else if (c < 2 || (code == 2 && c <= 5)) {   /* :1162-1166, 1365-1369 */
+         for (int a = 0; a <= na + 1; a++)
+            for (int b = 0; b <= nb + 1; b++)
+               P(a, b) = in[n++];
+      } else if (c <= 5) {                      /* code 1: :1167-1198 and twins */
+         for (int a = 0; a <= na + 1; a += (a == 0 || a == na + 1) ? 1 : 2) {
+            int a1 = (a == 0 || a == na + 1) ? a : a + 1;
+            for (int b = 0; b <= nb + 1; b += (b == 0 || b == nb + 1) ? 1 : 2, n++) {
+               int b1 = (b == 0 || b == nb + 1) ? b : b + 1;
+               for (int x = a; x <= a1; x++)
+                  for (int y = b; y <= b1; y++)
+                     P(x, y) = in[n];
+            }
+         }
+      } else {                                  /* codes 1, 2: :1199-1218 and twins */
+         for (int a = lo_a ? 0 : ha + 1; a <= (lo_a ? ha : na + 1); a++)
+            for (int b = lo_b ? 0 : hb + 1; b <= (lo_b ? hb : nb + 1); b++)
+               P(a, b) = in[n++];
+      }
+#undef P
+   }
+   return n;
+}
+
+int amr_dev_face_msg_len(amr_dev *d, int code, int dir, int face_case, int unpack)
+{
+   static double sink[2 * 1024 * 1024];
+   if (!d || code < 0 || code > 2 || dir < 0 || dir > 2 || face_case < 0 || face_case > 19)
+      return fail(d, AMR_ERR_ARG, "face_msg_len: bad arguments");
+   if (d->c.nx > 120 || d->c.ny > 120 || d->c.nz > 120)
+      return fail(d, AMR_ERR_ARG, "face_msg_len: blocks above 120 cells per axis");
+   if ((size_t) (d->gx + 2) * (d->gy + 2) * 4 > sizeof(sink) / sizeof(double))
+      return fail(d, AMR_ERR_ARG, "face_msg_len: block too large for this CPU library");
+   /* run the format on slot 0, variable 0 and count (unpack writes the
+    * values back, leaving the cells as they were) */
+   if (!unpack)
+      return (int) pack_one(d, code, dir, face_case, 0, 0, 1, sink);
+   double *save = (double *) malloc(sizeof(double) * d->slab);
+   memcpy(save, &A(d, 0, 0, 0, 0, 0), sizeof(double) * d->slab);
+   long n = unpack_one(d, code, dir, face_case, 0, 0, 1, sink);
+   memcpy(&A(d, 0, 0, 0, 0, 0), save, sizeof(double) * d->slab);
+   free(save);
+   return (int) n;
+}
+
+static int codec_args(amr_dev *d, int code, int dir, int start, int num, const amr_face_msg *msgs,
+                      int n, const void *buf, size_t buf_elems, int unpack)
+{
+   if (!d || code < 0 || code > 2 || dir < 0 || dir > 2 || start < 0 || num < 1 ||
+       start + num > d->c.num_vars || n < 0 || (n && (!msgs || !buf)))
+      return fail(d, AMR_ERR_ARG, "pack/unpack_faces: bad arguments");
+   for (int t = 0; t < n; t++) {
+      if (msgs[t].slot < 0 || msgs[t].slot >= d->c.max_blocks || msgs[t].face_case < 0 ||
+          msgs[t].face_case > 19 || msgs[t].offset < 0)
+         return fail(d, AMR_ERR_ARG, "pack/unpack_faces: bad message");
+      int len = amr_dev_face_msg_len(d, code, dir, msgs[t].face_case, unpack);
+      if ((size_t) msgs[t].offset + (size_t) num * len > buf_elems)
+         return fail(d, AMR_ERR_ARG, "pack/unpack_faces: message ends past the buffer");
+   }
+   return AMR_OK;
+}
+
+int amr_dev_pack_faces(amr_dev *d, int code, int dir, int start, int num_comm,
+                       const amr_face_msg *msgs, int n, void *buf, size_t buf_elems)
+{
+   int rc = codec_args(d, code, dir, start, num_comm, msgs, n, buf, buf_elems, 0);
+   for (int t = 0; !rc && t < n; t++)
+      pack_one(d, code, dir, msgs[t].face_case, msgs[t].slot, start, num_comm,
+               (double *) buf + msgs[t].offset);
+   return rc;
+}
+
+int amr_dev_unpack_faces(amr_dev *d, int code, int dir, int start, int num_comm,
+                         const amr_face_msg *msgs, int n, const void *buf, size_t buf_elems)
+{
+   int rc = codec_args(d, code, dir, start, num_comm, msgs, n, buf, buf_elems, 1);
+   for (int t = 0; !rc && t < n; t++)
+      unpack_one(d, code, dir, msgs[t].face_case, msgs[t].slot, start, num_comm,
+                 (const double *) buf + msgs[t].offset);
+   return rc;
+}
+
+/* Largest |a - b| over the active blocks' interior cells, per variable
+ * (FP64 contexts only in this CPU library). */
+int amr_dev_compare_cells(amr_dev *a, amr_dev *b, int start, int num, double *max_abs)
+{
+   if (!a || !b || !max_abs || start < 0 || num < 1 || start + num > a->c.num_vars ||
+       a->c.nx != b->c.nx || a->c.ny != b->c.ny || a->c.nz != b->c.nz ||
+       a->c.num_vars != b->c.num_vars || a->n_active != b->n_active)
+      return fail(a, AMR_ERR_ARG, "compare_cells: bad arguments or different contexts");
+   for (int in = 0; in < a->n_active; in++)
+      if (a->sorted[in] != b->sorted[in])
+         return fail(a, AMR_ERR_STATE, "compare_cells: the contexts hold different meshes");
+   for (int v = start; v < start + num; v++) {
+      double m = 0.0;
+      for (int in = 0; in < a->n_active; in++) {
+         const int s = a->sorted[in];
+         for (int i = 1; i <= a->c.nx; i++)
+            for (int j = 1; j <= a->c.ny; j++)
+               for (int k = 1; k <= a->c.nz; k++) {
+                  double d = fabs(A(a, v, s, i, j, k) - A(b, v, s, i, j, k));
+                  if (d > m || d != d)
+                     m = d != d ? INFINITY : d;
+               }
+      }
+      max_abs[v - start] = m;
+   }
+   return AMR_OK;
+}

@@ -18,6 +18,12 @@
  *                          the sequential (i,j,k) sum of the interior cells
  *                          and of the ghost cells (meshes too big to dump).
  *
+ *   AMR_DUMP_SAMPLE=<file> at the same point, 16384 cells drawn from a
+ *                          splitmix64 sequence (seed 0x5eed5eed5eed): per
+ *                          cell the sorted-list index, variable, i, j, k
+ *                          (ghosts included) and value.  The product's
+ *                          driver writes the same draws (host/driver.c).
+ *
  * Cell dump format (little-endian):
  *   int32 magic 0x414d5231 ('AMR1'), nx, ny, nz, num_vars, n_active,
  *         num_refine, pad
@@ -26,6 +32,9 @@
  * Block-sum format: same header with magic 0x414d5332 ('AMS2'), then per
  *   block: int64 number, int32 level, int32 slot,
  *          double sums[num_vars][2]   (interior, ghost)
+ * Sample format: same header with magic 0x414d5333 ('AMS3') and the cell
+ *   count in the last word, then per cell: int32 in, var, i, j, k, pad;
+ *   double value.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -92,6 +101,42 @@ static void dump_block_sums(const char *path)
    fclose(f);
 }
 
+#define SAMPLE_CELLS 16384
+
+static uint64_t splitmix64(uint64_t *x)
+{
+   uint64_t z = (*x += 0x9e3779b97f4a7c15ULL);
+   z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+   z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+   return z ^ (z >> 31);
+}
+
+static void dump_sample(const char *path)
+{
+   FILE *f = fopen(path, "wb");
+   if (!f) { perror(path); return; }
+   int32_t hdr[8];
+   int n_active = sorted_index[num_refine + 1];
+   hdr[0] = 0x414d5333; hdr[1] = x_block_size; hdr[2] = y_block_size;
+   hdr[3] = z_block_size; hdr[4] = num_vars; hdr[5] = n_active;
+   hdr[6] = num_refine; hdr[7] = n_active > 0 ? SAMPLE_CELLS : 0;
+   fwrite(hdr, sizeof(hdr), 1, f);
+   uint64_t x = 0x5eed5eed5eedULL;
+   for (int s = 0; s < hdr[7]; s++) {
+      int32_t r[6];
+      r[0] = (int32_t) (splitmix64(&x) % (uint64_t) n_active);
+      r[1] = (int32_t) (splitmix64(&x) % (uint64_t) num_vars);
+      r[2] = (int32_t) (splitmix64(&x) % (uint64_t) (x_block_size + 2));
+      r[3] = (int32_t) (splitmix64(&x) % (uint64_t) (y_block_size + 2));
+      r[4] = (int32_t) (splitmix64(&x) % (uint64_t) (z_block_size + 2));
+      r[5] = 0;
+      double v = blocks[sorted_list[r[0]].n].array[r[1]][r[2]][r[3]][r[4]];
+      fwrite(r, sizeof(r), 1, f);
+      fwrite(&v, sizeof(v), 1, f);
+   }
+   fclose(f);
+}
+
 int MPI_Allreduce(const void *sendbuf, void *recvbuf, int count,
                   MPI_Datatype datatype, MPI_Op op, MPI_Comm comm)
 {
@@ -112,9 +157,11 @@ int MPI_Allreduce(const void *sendbuf, void *recvbuf, int count,
    } else if (count == 142 && datatype == MPI_DOUBLE && !cells_done) {
       const char *p = getenv("AMR_DUMP_CELLS");
       const char *q = getenv("AMR_DUMP_BSUMS");
+      const char *r = getenv("AMR_DUMP_SAMPLE");
       cells_done = 1;
       if (p) dump_cells(p);
       if (q) dump_block_sums(q);
+      if (r) dump_sample(r);
    }
    return rc;
 }

@@ -65,9 +65,25 @@ def read_dump(path):
             key: np.stack(arrs) if arrs else np.zeros((0,) + tuple(shape))}
 
 
-def run(exe, args, dump=None, env=None, timeout=600):
-    """Run `exe args`; dump is None, "cells" or "bsums".  Returns a record
-    with the same keys as the golden JSON plus the dumps."""
+def read_sample(path):
+    """The AMR_DUMP_SAMPLE format: header of 8 int32 (magic 'AMS3', nx, ny,
+    nz, num_vars, n_active, num_refine, count), then per cell int32 in,
+    var, i, j, k, pad and a double value."""
+    with open(path, "rb") as f:
+        data = f.read()
+    hdr = np.frombuffer(data, dtype="<i4", count=8)
+    assert hdr[0] == 0x414d5333, hex(hdr[0])
+    rec = np.frombuffer(data, dtype=np.dtype([("idx", "<i4", 6), ("val", "<f8")]),
+                        count=int(hdr[7]), offset=32)
+    return {"in": rec["idx"][:, 0].copy(), "var": rec["idx"][:, 1].copy(),
+            "ijk": rec["idx"][:, 2:5].copy(), "val": rec["val"].copy(),
+            "n_active": int(hdr[5])}
+
+
+def run(exe, args, dump=None, env=None, timeout=600, sample=False):
+    """Run `exe args`; dump is None, "cells", "bsums" or "bsums_only";
+    sample=True adds the seeded cell sample (AMR_DUMP_SAMPLE).  Returns a
+    record with the same keys as the golden JSON plus the dumps."""
     with tempfile.TemporaryDirectory() as td:
         e = dict(os.environ)
         if env:
@@ -77,6 +93,8 @@ def run(exe, args, dump=None, env=None, timeout=600):
             e["AMR_DUMP_CELLS"] = os.path.join(td, "cells.bin")
         if dump in ("bsums", "bsums_only"):
             e["AMR_DUMP_BSUMS"] = os.path.join(td, "bsums.bin")
+        if sample:
+            e["AMR_DUMP_SAMPLE"] = os.path.join(td, "sample.bin")
         p = subprocess.run([exe] + args.split(), cwd=td, env=e, stdout=subprocess.PIPE,
                            stderr=subprocess.STDOUT, timeout=timeout)
         out = p.stdout.decode(errors="replace")
@@ -97,16 +115,19 @@ def run(exe, args, dump=None, env=None, timeout=600):
             rec["cells"] = read_dump(e["AMR_DUMP_CELLS"])
         if dump in ("bsums", "bsums_only") and os.path.exists(e["AMR_DUMP_BSUMS"]):
             rec["bsums"] = read_dump(e["AMR_DUMP_BSUMS"])
+        if sample and os.path.exists(e["AMR_DUMP_SAMPLE"]):
+            rec["sample"] = read_sample(e["AMR_DUMP_SAMPLE"])
         return rec
 
 
 BATCH_RUN_RE = re.compile(r"^run (\d+): ")
 
 
-def run_batch(exe, runs, dump=None, env=None, timeout=600, flags=()):
+def run_batch(exe, runs, dump=None, env=None, timeout=600, flags=(), sample=False):
     """Run a batch executable on the command lines `runs` (one string per
-    run); dump is None or "cells".  Returns {"rc", "stdout", "runs"} where
-    runs[i] has the keys of run()'s record for run i."""
+    run); dump is None or "cells"; sample=True adds each run's seeded cell
+    sample.  Returns {"rc", "stdout", "runs"} where runs[i] has the keys of
+    run()'s record for run i."""
     with tempfile.TemporaryDirectory() as td:
         path = os.path.join(td, "runs.txt")
         with open(path, "w") as f:
@@ -117,6 +138,8 @@ def run_batch(exe, runs, dump=None, env=None, timeout=600, flags=()):
         e["AMR_DUMP_SUMS"] = os.path.join(td, "sums")
         if dump == "cells":
             e["AMR_DUMP_CELLS"] = os.path.join(td, "cells")
+        if sample:
+            e["AMR_DUMP_SAMPLE"] = os.path.join(td, "sample")
         p = subprocess.run([exe] + list(flags) + [path], cwd=td, env=e, stdout=subprocess.PIPE,
                            stderr=subprocess.STDOUT, timeout=timeout)
         out = p.stdout.decode(errors="replace")
@@ -130,6 +153,9 @@ def run_batch(exe, runs, dump=None, env=None, timeout=600, flags=()):
             cp = "%s.%d" % (e.get("AMR_DUMP_CELLS", ""), i)
             if dump == "cells" and os.path.exists(cp):
                 rec["cells"] = read_dump(cp)
+            qp = "%s.%d" % (e.get("AMR_DUMP_SAMPLE", ""), i)
+            if sample and os.path.exists(qp):
+                rec["sample"] = read_sample(qp)
             recs.append(rec)
         cur = None   # "run i: " opens run i's report; its Summary line follows
         for line in out.splitlines():
@@ -232,6 +258,14 @@ def compare(rec, gold, sum_rtol=0.0, max_ulp=0, check_cells=True, cell_vars=None
         e = rel_err(mine[..., 0], g["sums"][..., 0])
         report["max_bsum_rel"] = float(e.max())
         assert e.max() <= max(sum_rtol, 1e-12), "block interior sum rel err %.3e" % e.max()
+        if g["sums"].shape[-1] > 1 and "--stencil 27" in gold["args"]:
+            # ghost sums: every ghost cell of a 27-point run is one the
+            # exchange wrote or a slot's earlier value, as in the reference
+            # (absolute scale 1: a sum of the reference's never-written
+            # fresh ghosts is 0 or heap debris of order 1e-300)
+            d = np.abs(mine[..., 1] - g["sums"][..., 1]) / np.maximum(np.abs(g["sums"][..., 1]), 1.0)
+            report["max_ghost_bsum_rel"] = float(d.max())
+            assert d.max() <= max(sum_rtol, 1e-12), "block ghost sum rel err %.3e" % d.max()
         if "cells" in rec and "sample_block" in g:
             cells = rec["cells"]["cells"]
             bi, vi, idx = g["sample_block"], g["sample_var"], g["sample_idx"]
@@ -239,4 +273,14 @@ def compare(rec, gold, sum_rtol=0.0, max_ulp=0, check_cells=True, cell_vars=None
             u = ulp_diff(vals, g["sample_val"])
             report["max_sample_ulp"] = int(u.max())
             assert u.max() <= max_ulp, "sampled cell ulp diff %d" % u.max()
+    if check_cells and "bsums" in gold and "sample_in" in gold["bsums"]:
+        # the seeded AMR_DUMP_SAMPLE cells (ghosts included)
+        g, smp = gold["bsums"], rec["sample"]
+        assert smp["n_active"] == len(g["number"])
+        assert np.array_equal(smp["in"], g["sample_in"]) and np.array_equal(smp["var"], g["sample_var"])
+        assert np.array_equal(smp["ijk"], g["sample_idx"])
+        u = ulp_diff(smp["val"], g["sample_val"])
+        report["max_sample_ulp"] = int(u.max())
+        report["sampled_cells"] = int(u.size)
+        assert u.max() <= max_ulp, "sampled cell ulp diff %d" % u.max()
     return report

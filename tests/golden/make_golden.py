@@ -15,7 +15,10 @@ directory:
 * ``<case>.cells.npz`` -- for tiny meshes, every active block's full cell
   array (ghosts included) at the end of the run;
 * ``<case>.bsums.npz`` -- for larger meshes, per (block, variable) interior
-  and ghost sums at the end of the run, plus a seeded sample of cells.
+  and ghost sums at the end of the run, plus a seeded sample of cells (for
+  the bandwidth-scale workload: all 40 variables' sums and the 16,384-cell
+  AMR_DUMP_SAMPLE draws, ghosts included, which the product's driver
+  reproduces).
 
 Only this container has ``/root/reference``; the fixtures are what travels.
 Usage: ``python tests/golden/make_golden.py [case ...]``.
@@ -114,12 +117,13 @@ CASES = {
     "err_flag": ("--nx 4 --bogus 3", None),
     # The bandwidth-scale workload (27-point, 4 levels, two objects).  The
     # serial reference needs ~20 minutes for it, so it is only generated
-    # when named explicitly; per-block sums are kept for 3 variables.
+    # when named explicitly; per-block interior and ghost sums of all 40
+    # variables and the seeded 16,384-cell sample (ghosts included).
     "cfg1": ("--init_x 4 --init_y 4 --init_z 4 --nx 16 --ny 16 --nz 16 --num_vars 40 "
              "--stencil 27 --num_tsteps 10 --stages_per_ts 20 --checksum_freq 5 --num_refine 4 "
              "--refine_freq 4 --max_blocks 8000 --error_tol -10 --num_objects 2 "
              "--object 2 0 0.15 0.15 0.15 0.02 0.02 0.02 0.1 0.1 0.1 0 0 0 "
-             "--object 0 0 0.5 0.5 0.5 0.02 0.02 0.02 0.05 0.05 0.05 0 0 0", "bsums_only"),
+             "--object 0 0 0.5 0.5 0.5 0.02 0.02 0.02 0.05 0.05 0.05 0 0 0", "bsums_sample"),
     "err_odd": ("--nx 5 --num_tsteps 1", None),
 }
 
@@ -173,6 +177,9 @@ def run_case(name, args, kind, outdir=HERE):
             env["AMR_DUMP_BSUMS"] = os.path.join(td, "bsums.bin")
         elif kind == "bsums_only":
             env["AMR_DUMP_BSUMS"] = os.path.join(td, "bsums.bin")
+        elif kind == "bsums_sample":
+            env["AMR_DUMP_BSUMS"] = os.path.join(td, "bsums.bin")
+            env["AMR_DUMP_SAMPLE"] = os.path.join(td, "sample.bin")
         p = subprocess.run([REF_DUMP] + args.split(), cwd=td, env=env,
                            stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
                            timeout=3600)
@@ -229,6 +236,9 @@ def run_case(name, args, kind, outdir=HERE):
                 sample_val=vals)
         if kind == "bsums_only" and os.path.exists(env["AMR_DUMP_BSUMS"]):
             save_bsums_subset(name, read_dump(env["AMR_DUMP_BSUMS"]), outdir)
+        if kind == "bsums_sample" and os.path.exists(env["AMR_DUMP_BSUMS"]):
+            save_bsums_sample(name, read_dump(env["AMR_DUMP_BSUMS"]),
+                              read_sample(env["AMR_DUMP_SAMPLE"]), outdir)
         with open(os.path.join(outdir, name + ".json"), "w") as f:
             json.dump(rec, f, indent=0)
         return rec
@@ -247,10 +257,36 @@ def save_bsums_subset(name, d, outdir=HERE):
         sums=np.stack([b[3][BSUM_VARS] for b in d["blocks"]]))
 
 
+def read_sample(path):
+    """AMR_DUMP_SAMPLE: 8 int32 header (count last), then per cell int32 in,
+    var, i, j, k, pad and a double."""
+    with open(path, "rb") as f:
+        data = f.read()
+    hdr = struct.unpack_from("<8i", data, 0)
+    assert hdr[0] == 0x414d5333
+    rec = np.frombuffer(data, dtype=np.dtype([("idx", "<i4", 6), ("val", "<f8")]),
+                        count=hdr[7], offset=32)
+    return rec
+
+
+def save_bsums_sample(name, d, smp, outdir=HERE):
+    np.savez_compressed(
+        os.path.join(outdir, name + ".bsums.npz"),
+        number=np.array([b[0] for b in d["blocks"]], dtype=np.int64),
+        level=np.array([b[1] for b in d["blocks"]], dtype=np.int32),
+        slot=np.array([b[2] for b in d["blocks"]], dtype=np.int32),
+        vars=np.arange(d["num_vars"], dtype=np.int32),
+        sums=np.stack([b[3] for b in d["blocks"]]),
+        sample_in=smp["idx"][:, 0].astype(np.int32),
+        sample_var=smp["idx"][:, 1].astype(np.int32),
+        sample_idx=smp["idx"][:, 2:5].astype(np.int32),
+        sample_val=smp["val"].copy())
+
+
 def main(names):
     if not os.path.exists(REF_DUMP):
         sys.exit("build the reference first: make -C oracle ref")
-    for name in names or [c for c in CASES if CASES[c][1] != "bsums_only"]:
+    for name in names or [c for c in CASES if CASES[c][1] not in ("bsums_only", "bsums_sample")]:
         args, kind = CASES[name]
         rec = run_case(name, args, kind)
         print("%-12s rc=%d sums=%d counts=%d msgs=%s" % (

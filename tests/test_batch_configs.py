@@ -77,3 +77,29 @@ def test_block_counts_in_range():
         assert hw <= int(opt(r, "--max_blocks")), (r, hw)
         counts += tot
     assert max(counts) / min(counts) > 10   # uneven on purpose
+
+
+def test_fp64_fixtures_reproduce():
+    """tests/golden/batch64_f64.npz is what the CPU oracle gives for the
+    table's command lines in FP64 (make_batch_golden.py): re-run the four
+    smallest runs and compare checksums, block counts and samples exactly."""
+    import numpy as np
+    require(A.ORACLE_X, "run __graft_entry__.build()")
+    gold = np.load(os.path.join(A.GOLDEN, "batch64_f64.npz"))
+    runs = table()
+    small = sorted(range(64), key=lambda i: gold["blocks_per_ts"][i] *
+                   int(opt(runs[i], "--nx")) ** 3 * int(opt(runs[i], "--num_vars")))[:4]
+
+    def one(i):
+        return i, A.run(A.ORACLE_X, runs[i].replace("--precision f32", "--precision f64"),
+                        timeout=300, sample=True)
+    with ThreadPoolExecutor(4) as ex:
+        res = list(ex.map(one, small))
+    off, coff = gold["sums_off"], gold["counts_off"]
+    for i, r in res:
+        assert r["rc"] == 0
+        assert r["sums"] == list(gold["sums"][off[i]:off[i + 1]]), i
+        assert r["counts"] == gold["counts"][coff[i]:coff[i + 1]].tolist(), i
+        assert r["summary"]["max_blocks"] == gold["max_blocks"][i]
+        k = gold["sample_val"].shape[1]
+        assert np.array_equal(r["sample"]["val"][:k], gold["sample_val"][i]), i

@@ -38,17 +38,49 @@ def test_fused_stages_are_identical(name):
     kind = "cells" if "cells" in g else ("bsums" if "bsums" in g else None)
     if kind == "bsums" and "vars" in g["bsums"]:
         kind = "bsums_only"
-    a = A.run(M.EXE_PATH, g["args"], dump=kind, timeout=110, env={"MINIAMR_FUSED": "0"})
-    b = A.run(M.EXE_PATH, g["args"], dump=kind, timeout=110, env=FUSED)
+    sample = "bsums" in g and "sample_in" in g["bsums"]
+    a = A.run(M.EXE_PATH, g["args"], dump=kind, timeout=110, env={"MINIAMR_FUSED": "0"},
+              sample=sample)
+    b = A.run(M.EXE_PATH, g["args"], dump=kind, timeout=110, env=FUSED, sample=sample)
+    assert_same(a, b)
+    check_cells = not any("difference too large" in m for m in g["messages"])
+    A.compare(b, g, sum_rtol=1e-12, max_ulp=4, check_cells=check_cells)
+
+
+def assert_same(a, b):
     assert a["rc"] == b["rc"]
     assert a["sums"] == b["sums"]
     assert a["counts"] == b["counts"]
-    for k in ("cells", "bsums"):
+    for k in ("cells", "bsums", "sample"):
         if k in a:
             for key in a[k]:
                 assert np.array_equal(a[k][key], b[k][key]), (k, key)
-    check_cells = not any("difference too large" in m for m in g["messages"])
-    A.compare(b, g, sum_rtol=1e-12, max_ulp=4, check_cells=check_cells)
+
+
+# The narrow storage forms: fused and plain stages must agree bit for bit
+# in every precision (the ghosts are exact copies), with variable counts
+# that leave partial stencil groups (2, 3, 5 variables against 4 per
+# workgroup) and partial 16-variable exchange items, the bandwidth-scale
+# workload itself (40 variables, fused by default), and the launch-shape
+# overrides.
+NARROW = [("t27_a", None), ("t27_cv", None), ("t7_r3", None), ("cfg4", None), ("cfg1", None),
+          ("t27_cv", {"MINIAMR_STENCIL_VPW": "1", "MINIAMR_EXCHANGE": "16,256"}),
+          ("cfg4", {"MINIAMR_STENCIL_VPW": "2", "MINIAMR_EXCHANGE": "4,128"})]
+
+
+@pytest.mark.parametrize("prec", ["f32", "bf16"])
+@pytest.mark.parametrize("name,extra", NARROW,
+                         ids=["%s%s" % (n, "-override" if e else "") for n, e in NARROW])
+def test_fused_stages_are_identical_in_narrow_storage(name, extra, prec):
+    require(M.EXE_PATH, "run __graft_entry__.build()")
+    g = A.golden(name)
+    print("case: fused %s %s %s" % (prec, name, extra or ""), flush=True)
+    kind = "cells" if "cells" in g else "bsums_only"
+    env = dict(extra or {}, MINIAMR_PRECISION=prec)
+    a = A.run(M.EXE_PATH, g["args"], dump=kind, timeout=110, env=dict(env, MINIAMR_FUSED="0"))
+    b = A.run(M.EXE_PATH, g["args"], dump=kind, timeout=110, env=dict(env, **FUSED))
+    assert len(a["sums"]) > 0
+    assert_same(a, b)
 
 
 def test_stale_ghosts_are_refused(monkeypatch):

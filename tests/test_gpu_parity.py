@@ -179,8 +179,9 @@ def test_driver_matches_reference(name):
     if kind == "bsums" and "vars" in g["bsums"]:
         kind = "bsums_only"        # large mesh: per-block sums only
     # refinement marked on the device (fails loudly if the device declines)
+    sample = "bsums" in g and "sample_in" in g["bsums"]
     r = A.run(M.EXE_PATH, g["args"], dump=kind, timeout=110,
-              env={"MINIAMR_DEVICE_MARKING": "1"})
+              env={"MINIAMR_DEVICE_MARKING": "1"}, sample=sample)
     check_cells = not any("difference too large" in m for m in g["messages"])
     rep = A.compare(r, g, sum_rtol=1e-12, max_ulp=4, check_cells=check_cells)
     print(name, rep)
