@@ -62,6 +62,31 @@ struct FusedJob {
    int32_t pad;
 };
 
+// Where a fused stage finds the cells of one ghost edge line or corner of a
+// block: kEdgeSrcs entries per slot, in this order
+//   0..3    lines along z, cells (x ghost, y ghost, k = 1..nz), xs*2 + ys
+//   4..7    lines along y, cells (x ghost, j = 1..ny, z ghost), 4 + xs*2 + zs
+//   8..11   lines along x, cells (i = 1..nx, y ghost, z ghost), 8 + ys*2 + zs
+//   12..19  corners, 12 + xs*4 + ys*2 + zs
+// (xs = 0 for x = 0, 1 for x = nx+1, and so on).  Cell t of a line (t = 0
+// for the first) is at element slot*rec + off + t*stride of the variable,
+// plus the offset of the interior copy being read for kind ES_I (an
+// interior cell of slot) or of its z-plane copy for ES_ZI.  With edge
+// folding an entry names the cell the 27-point extension of the exchange
+// would have copied (the copy is then not made); otherwise the block's own
+// ghost cell.
+enum { ES_ABS = 0, ES_I = 1, ES_ZI = 2 };
+struct EdgeSrc {
+   int32_t slot, off, stride, kind;
+};
+constexpr int kEdgeSrcs = 20;
+
+__host__ __device__ __forceinline__ int64_t edge_at(const EdgeSrc &e, const Lay &L, int t)
+{
+   const int part = e.kind == ES_I ? L.oI : e.kind == ES_ZI ? L.oZI : 0;
+   return (int64_t) e.slot * L.rec + e.off + part + (int64_t) t * e.stride;
+}
+
 // Recorded data operations of a context (include/amr_adjoint.h): each op
 // is {kind, args..., plan in force}; plans and job lists are host copies.
 struct TapePlan {
@@ -149,6 +174,11 @@ struct amr_dev {
    int32_t fcount[3] = {0, 0, 0};
    int fjobs_cap = 0;
    int32_t *d_redir = nullptr;
+   // kEdgeSrcs entries per slot (EdgeSrc); fold_edges: the plan drops the
+   // 27-point edge extensions of in-place faces (the stencil and the
+   // remaining jobs read their cells where the extension would copy from)
+   EdgeSrc *d_edges = nullptr;
+   bool fold_edges = false;
    std::vector<uint8_t> stale;
 
    Tape *tape = nullptr;            // recording of data operations, if any

@@ -177,6 +177,7 @@ int amr_dev_destroy(amr_dev *d)
    hipFree(d->d_cmp);
    hipFree(d->d_fjobs);
    hipFree(d->d_redir);
+   hipFree(d->d_edges);
    hipFree(d->d_a0);
    hipFree(d->d_check_counts);
    hipFree(d->d_matwork);
@@ -255,6 +256,146 @@ Lay amr_lay_other(const amr_dev *d, const Lay &L)
    return o;
 }
 
+// MINIAMR_FOLD_EDGES=0 keeps the 27-point edge extensions of in-place faces
+// as exchange jobs (the default folds them into the reads, below).
+static bool fold_edges_on()
+{
+   static int v = -1;
+   if (v < 0) {
+      const char *e = getenv("MINIAMR_FOLD_EDGES");
+      v = e ? atoi(e) != 0 : 1;
+   }
+   return v != 0;
+}
+
+// Edge sources of a fused stage (EdgeSrc).  In the reference a ghost edge
+// cell of a block gets its value from the last job that writes it in a
+// stage (27-point stencil, --code 0; ref/comm.c:1597-1688): a cell with a z
+// ghost coordinate from the block's z-face job, which copies the z-plane
+// ring of its source; a cell with x and y ghost coordinates (z interior)
+// from the y-face job, which copies the source's x-extended rows; each
+// copies the source's cell at the same in-plane coordinates on the plane
+// next to the shared face.  Only same-level jobs of blocks without a
+// boundary face extend and are folded (a boundary block's jobs all run);
+// a prolongation or restriction writes no edge cell, which then keeps what
+// the slot held (stale).  Following these copies, and the in-place face
+// reads of the plan, back to a cell that some job in the stage does write
+// or that is an interior cell gives the value the stage's stencil sees.
+namespace {
+
+struct Cell { int slot, i, j, k; };
+
+struct EdgeResolver {
+   const Lay &g;
+   const std::vector<int32_t> &same;
+   const std::vector<uint8_t> &bc;
+
+   // the cell whose value block b's cell (i, j, k) holds when the stencil runs
+   Cell resolve(int b, int i, int j, int k) const
+   {
+      for (int hop = 0; hop < 8; hop++) {
+         const bool gi = i == 0 || i == g.nx + 1, gj = j == 0 || j == g.ny + 1,
+                    gk = k == 0 || k == g.nz + 1;
+         const int ng = gi + gj + gk;
+         if (!ng || bc[b])
+            return {b, i, j, k};
+         int f, src;
+         if (ng == 1)           // face interior: read in place, or the job writes it
+            f = gi ? (i != 0) : gj ? 2 + (j != 0) : 4 + (k != 0);
+         else if (gk)           // z ring: the z-face job's extension
+            f = 4 + (k != 0);
+         else                   // x and y ghost, z interior: the y-face job's extension
+            f = 2 + (j != 0);
+         src = same[6 * (size_t) b + f];
+         if (src < 0)
+            return {b, i, j, k};
+         b = src;
+         if (f < 2) i = (f & 1) ? 1 : g.nx;
+         else if (f < 4) j = (f & 1) ? 1 : g.ny;
+         else k = (f & 1) ? 1 : g.nz;
+      }
+      return {-1, 0, 0, 0};
+   }
+
+   // a cell as an entry: axis is the direction of the line (-1 for a corner);
+   // cells at z = 1 / z = nz of lines across z are read from the z-plane copies
+   bool entry(const Cell &c, int axis, EdgeSrc &e) const
+   {
+      if (c.slot < 0)
+         return false;
+      const bool ghost = c.i == 0 || c.i == g.nx + 1 || c.j == 0 || c.j == g.ny + 1 ||
+                         c.k == 0 || c.k == g.nz + 1;
+      const bool zc = axis != 2 && (c.k == 1 || c.k == g.nz);
+      Lay L0 = g;
+      L0.oI = 0;
+      L0.oZI = 0;
+      e.slot = c.slot;
+      e.stride = 0;
+      if (!ghost) {
+         e.kind = zc ? ES_ZI : ES_I;
+         e.off = zc ? zcopy(L0, c.i, c.j, c.k) : home(L0, c.i, c.j, c.k);
+      } else {
+         e.kind = ES_ABS;
+         e.off = zc ? zcopy(g, c.i, c.j, c.k) : home(g, c.i, c.j, c.k);
+      }
+      return true;
+   }
+
+   // entry of a line of n cells, cell(t) giving block b's cell t (resolved
+   // when folding, else the block's own); false if the cells are not one
+   // affine run of one block and kind
+   template <typename F>
+   bool line(int b, int axis, int n, F cell, bool fold, EdgeSrc &e) const
+   {
+      auto at = [&](int t, EdgeSrc &x) {
+         const Cell c = cell(t);
+         return entry(fold ? resolve(b, c.i, c.j, c.k) : c, axis, x);
+      };
+      if (!at(0, e))
+         return false;
+      if (n > 1) {
+         EdgeSrc e1;
+         if (!at(1, e1) || e1.slot != e.slot || e1.kind != e.kind)
+            return false;
+         e.stride = e1.off - e.off;
+         for (int t = 2; t < n; t++) {
+            EdgeSrc et;
+            if (!at(t, et) || et.slot != e.slot || et.kind != e.kind ||
+                et.off != e.off + t * e.stride)
+               return false;
+         }
+      }
+      return true;
+   }
+
+   // the kEdgeSrcs entries of block b
+   bool block(int b, bool fold, EdgeSrc *out) const
+   {
+      const int X[2] = {0, g.nx + 1}, Y[2] = {0, g.ny + 1}, Z[2] = {0, g.nz + 1};
+      bool ok = true;
+      for (int xs = 0; xs < 2; xs++)
+         for (int ys = 0; ys < 2; ys++)
+            ok &= line(b, 2, g.nz, [&](int t) { return Cell{b, X[xs], Y[ys], t + 1}; }, fold,
+                       out[xs * 2 + ys]);
+      for (int xs = 0; xs < 2; xs++)
+         for (int zs = 0; zs < 2; zs++)
+            ok &= line(b, 1, g.ny, [&](int t) { return Cell{b, X[xs], t + 1, Z[zs]}; }, fold,
+                       out[4 + xs * 2 + zs]);
+      for (int ys = 0; ys < 2; ys++)
+         for (int zs = 0; zs < 2; zs++)
+            ok &= line(b, 0, g.nx, [&](int t) { return Cell{b, t + 1, Y[ys], Z[zs]}; }, fold,
+                       out[8 + ys * 2 + zs]);
+      for (int xs = 0; xs < 2; xs++)
+         for (int ys = 0; ys < 2; ys++)
+            for (int zs = 0; zs < 2; zs++)
+               ok &= line(b, -1, 1, [&](int) { return Cell{b, X[xs], Y[ys], Z[zs]}; }, fold,
+                          out[12 + xs * 4 + ys * 2 + zs]);
+      return ok;
+   }
+};
+
+}  // namespace
+
 // The plan of amr_dev_stage_fused.  A block's face is read in place when
 // its job is a same-level copy and the block has no boundary face (a
 // boundary fill with the 27-point extension reads the block's own ghost
@@ -278,6 +419,24 @@ static int build_fused_plan(amr_dev *d, const int32_t *sorted, int n_active,
          same[6 * (size_t) j.dst + j.face] = j.src;
    }
    auto inplace = [&](int slot, int f) { return bc[slot] ? -1 : same[6 * (size_t) slot + f]; };
+   // the in-place faces of each slot (those of boundary blocks are not)
+   std::vector<int32_t> inp(6 * (size_t) M, -1);
+   for (int in = 0; in < n_active; in++)
+      for (int f = 0; f < 6; f++)
+         inp[6 * (size_t) sorted[in] + f] = inplace(sorted[in], f);
+   // edge sources of every active block: folded (27-point stencil) unless a
+   // line is not one affine run, in which case the plan keeps the extensions
+   std::vector<EdgeSrc> edges(kEdgeSrcs * (size_t) M, EdgeSrc{0, 0, 0, ES_ABS});
+   bool fold = d->g.stencil == 27 && fold_edges_on();
+   {
+      EdgeResolver R{d->g, inp, bc};
+      for (int in = 0; in < n_active && fold; in++)
+         fold = R.block(sorted[in], true, &edges[kEdgeSrcs * (size_t) sorted[in]]);
+      if (!fold)
+         for (int in = 0; in < n_active; in++)
+            R.block(sorted[in], false, &edges[kEdgeSrcs * (size_t) sorted[in]]);
+   }
+   d->fold_edges = fold;
    std::vector<FusedJob> fj;
    fj.reserve(nj);
    int32_t fc[3] = {0, 0, 0};
@@ -290,8 +449,9 @@ static int build_fused_plan(amr_dev *d, const int32_t *sorted, int n_active,
          f.alt[0] = f.alt[1] = f.alt[2] = f.alt[3] = -1;
          f.pad = 0;
          if (j.kind == AMR_FACE_SAME && inplace(j.dst, j.face) >= 0) {
-            if (dir == 0 || d->g.stencil == 7)
-               continue;                    // nothing outside the face interior
+            if (dir == 0 || d->g.stencil == 7 || fold)
+               continue;                    // nothing outside the face interior, or
+                                            // the extension is read where it is
             f.mode = FJ_EXT_ONLY;
          }
          if (d->g.stencil == 27 && dir > 0 &&
@@ -320,6 +480,10 @@ static int build_fused_plan(amr_dev *d, const int32_t *sorted, int n_active,
    if (!fj.empty())
       HIP_TRY(d, hipMemcpy(d->d_fjobs, fj.data(), sizeof(FusedJob) * fj.size(),
                            hipMemcpyHostToDevice));
+   if (!d->d_edges)
+      HIP_TRY(d, hipMalloc(&d->d_edges, sizeof(EdgeSrc) * edges.size()));
+   HIP_TRY(d, hipMemcpy(d->d_edges, edges.data(), sizeof(EdgeSrc) * edges.size(),
+                        hipMemcpyHostToDevice));
    if (!d->d_redir)
       HIP_TRY(d, hipMalloc(&d->d_redir, sizeof(int32_t) * 6 * (size_t) M));
    HIP_TRY(d, hipMemcpy(d->d_redir, redir.data(), sizeof(int32_t) * redir.size(),

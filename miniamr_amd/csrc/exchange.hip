@@ -43,12 +43,13 @@ __global__ __launch_bounds__(NT) void k_exchange(T *__restrict__ pool,
 // The fused stage's jobs (FusedJob), same launch shape.
 template <typename T, int KV, int NT>
 __global__ __launch_bounds__(NT) void k_exchange_fused(T *__restrict__ pool,
-                                                       const FusedJob *__restrict__ jobs, Lay L,
+                                                       const FusedJob *__restrict__ jobs,
+                                                       const EdgeSrc *__restrict__ edges, Lay L,
                                                        int var0, int var_end)
 {
    const FusedJob fj = jobs[blockIdx.x];
    const int v0 = var0 + blockIdx.y * KV;
-   exchange_job_fused<T, KV>(pool, fj, L, v0, min(KV, var_end - v0), threadIdx.x, NT);
+   exchange_job_fused<T, KV>(pool, fj, edges, L, v0, min(KV, var_end - v0), threadIdx.x, NT);
 }
 
 // Variables and threads per workgroup.  Default 8 x 256 (measured best for
@@ -127,15 +128,18 @@ int launch_fused(amr_dev *d, int dir, int start, int num)
    if (kv == 16) {
       dim3 grid(n, (num + 15) / 16);
       hipLaunchKernelGGL((k_exchange_fused<T, 16, 256>), grid, dim3(256), 0, d->stream,
-                         (T *) d->pool, d->d_fjobs + off, L, start, start + num);
+                         (T *) d->pool, d->d_fjobs + off, d->d_edges, L, start,
+                         start + num);
    } else if (kv == 4) {
       dim3 grid(n, (num + 3) / 4);
       hipLaunchKernelGGL((k_exchange_fused<T, 4, 256>), grid, dim3(256), 0, d->stream,
-                         (T *) d->pool, d->d_fjobs + off, L, start, start + num);
+                         (T *) d->pool, d->d_fjobs + off, d->d_edges, L, start,
+                         start + num);
    } else {
       dim3 grid(n, (num + kVars - 1) / kVars);
       hipLaunchKernelGGL((k_exchange_fused<T, kVars, 256>), grid, dim3(256), 0, d->stream,
-                         (T *) d->pool, d->d_fjobs + off, L, start, start + num);
+                         (T *) d->pool, d->d_fjobs + off, d->d_edges, L, start,
+                         start + num);
    }
    HIP_TRY(d, hipGetLastError());
    return AMR_OK;

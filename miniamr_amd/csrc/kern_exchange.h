@@ -185,11 +185,14 @@ __device__ __forceinline__ void exchange_job(T *__restrict__ pool, const amr_fac
 // outside the face interior (two rows for y, the ring of the plane for z),
 // and a read of a source ghost row that is read in place (alt[] >= 0) goes
 // to the cell that row would hold: the x = 0 row of the source is the
-// interior x = nx plane of the block alt[0], and so on.  The source block
-// then varies per cell, so reads use 64-bit offsets from the variable's
-// base.
+// interior x = nx plane of the block alt[0], and so on.  A z job reads the
+// four corners of the source's ring (cells of its edge lines along z) where
+// the source's edge entries say (edges, kEdgeSrcs per slot).  The source
+// block then varies per cell, so reads use 64-bit offsets from the
+// variable's base.
 template <typename T, int KV = kVars>
 __device__ __forceinline__ void exchange_job_fused(T *__restrict__ pool, const FusedJob &fj,
+                                                   const EdgeSrc *__restrict__ edges,
                                                    const Lay &L, int v0, int nv, int tid,
                                                    int nthr)
 {
@@ -241,6 +244,7 @@ __device__ __forceinline__ void exchange_job_fused(T *__restrict__ pool, const F
       canon(dir, ghost, a, b, i, j, k);
       int slot = jb.src;
       int so = dir == 2 ? zcopy(L, si, sj, sk) : home(L, si, sj, sk);
+      int64_t src = -1;
       if (dir == 1) {
          if (si == 0 && fj.alt[0] >= 0) {
             slot = fj.alt[0];
@@ -263,9 +267,13 @@ __device__ __forceinline__ void exchange_job_fused(T *__restrict__ pool, const F
          } else if (sj == L.ny + 1 && xin && fj.alt[3] >= 0) {
             slot = fj.alt[3];
             so = zcopy(L, si, 1, sk);
+         } else if (!xin && !yin) {
+            src = edge_at(edges[(int64_t) kEdgeSrcs * jb.src + (si != 0) * 2 + (sj != 0)], L,
+                          sk - 1);
          }
       }
-      const int64_t src = (int64_t) slot * L.rec + so;
+      if (src < 0)
+         src = (int64_t) slot * L.rec + so;
       const int64_t dh = dst0 + home(L, i, j, k);
       const int64_t dc = (k == 1 || k == L.nz) ? dst0 + zcopy(L, i, j, k) : -1;
       T v[KV];

@@ -121,8 +121,9 @@ template <typename T> __device__ __forceinline__ T load_at(gptr<T> b, int64_t o)
 // in flight (small blocks can afford all of theirs).
 //
 // FUSED (a stage that reads same-level faces in place): the block's slot
-// bslot and redir[6], the slots its six faces are read from (-1: its own ghost
-// plane).  The face interior of a redirected face is read from the
+// bslot, redir[6], the slots its six faces are read from (-1: its own ghost
+// plane), and es[kEdgeSrcs], where its ghost edge lines and corners are
+// read (EdgeSrc).  The face interior of a redirected face is read from the
 // neighbour's interior (its z-plane copy for z faces) in the copy L
 // describes, and the outputs go to the copy Lout describes, so no
 // neighbour's cells change while they are read.  Plane offsets are then
@@ -134,7 +135,8 @@ __device__ __forceinline__ void stencil_block(T *__restrict__ rec, const Lay &L,
                                               typename Acc<T>::type *ring, double *red,
                                               double *__restrict__ partials, int part_stride,
                                               const Lay *Lout = nullptr, int bslot = 0,
-                                              const int32_t *__restrict__ redir = nullptr)
+                                              const int32_t *__restrict__ redir = nullptr,
+                                              const EdgeSrc *__restrict__ es = nullptr)
 {
    // interior cells are read with L and written with Lo: the same interior
    // copy (in place), or the record's other copy
@@ -194,15 +196,32 @@ __device__ __forceinline__ void stencil_block(T *__restrict__ rec, const Lay &L,
       } else {
          const int64_t own = (int64_t) bslot * L.rec;
          const bool jin = jj >= 1 && jj <= L.ny, kin = kk >= 1 && kk <= nz;
-         off0[q] = jin && kin && rdr[0] >= 0 ? (int64_t) rdr[0] * L.rec + home(L, L.nx, jj, kk)
-                                             : own + home(L, 0, jj, kk);
-         offN[q] = jin && kin && rdr[1] >= 0 ? (int64_t) rdr[1] * L.rec + home(L, 1, jj, kk)
-                                             : own + home(L, L.nx + 1, jj, kk);
+         if (jin && kin) {
+            off0[q] = rdr[0] >= 0 ? (int64_t) rdr[0] * L.rec + home(L, L.nx, jj, kk)
+                                  : own + home(L, 0, jj, kk);
+            offN[q] = rdr[1] >= 0 ? (int64_t) rdr[1] * L.rec + home(L, 1, jj, kk)
+                                  : own + home(L, L.nx + 1, jj, kk);
+         } else {
+            // an edge line or corner of the x = 0 / x = nx+1 planes (the
+            // x = nx+1 entry follows the x = 0 one at +2, corners at +4)
+            int e, t;
+            if (kin) { e = jj != 0; t = kk - 1; }
+            else if (jin) { e = 4 + (kk != 0); t = jj - 1; }
+            else { e = 12 + (jj != 0) * 2 + (kk != 0); t = 0; }
+            off0[q] = edge_at(es[e], L, t);
+            offN[q] = edge_at(es[e + (e < 12 ? 2 : 4)], L, t);
+         }
          // interior planes: a y / z face row from the neighbour's interior
-         // (z: its z-plane copy), else the block's own record
+         // (z: its z-plane copy), an edge line along x from its source, else
+         // the block's own record
          int o1 = home(L, 1, jj, kk), o2 = L.nx > 1 ? home(L, 2, jj, kk) : o1;
          int64_t b = own;
-         if (kin && jj == 0 && rdr[2] >= 0) {
+         if (!jin && !kin) {
+            const EdgeSrc &x = es[8 + (jj != 0) * 2 + (kk != 0)];
+            b = edge_at(x, L, 0);
+            o1 = 0;
+            o2 = x.stride;
+         } else if (kin && jj == 0 && rdr[2] >= 0) {
             b = (int64_t) rdr[2] * L.rec;
             o1 = home(L, 1, L.ny, kk);
             o2 = L.nx > 1 ? home(L, 2, L.ny, kk) : o1;

@@ -59,11 +59,13 @@ __global__ __launch_bounds__(kThreads) void k_stencil(T *__restrict__ pool,
 
 // The stencil of a fused stage: interiors read from the copy L describes,
 // written to the copy Lo describes, same-level faces read in place through
-// redir (six slots per active block, sorted order).
+// redir (six slots per active block, sorted order), ghost edge lines and
+// corners where edges (kEdgeSrcs entries per slot) says.
 template <typename T, int ST, int NC, int NP, int V, int NZ>
 __global__ __launch_bounds__(kThreads) void k_stencil_fused(T *__restrict__ pool,
                                                              const int32_t *__restrict__ sorted,
                                                              const int32_t *__restrict__ redir,
+                                                             const EdgeSrc *__restrict__ edges,
                                                              Lay L, Lay Lo, int var0, int var_end,
                                                              double *__restrict__ partials,
                                                              int part_stride, int remap)
@@ -83,7 +85,7 @@ __global__ __launch_bounds__(kThreads) void k_stencil_fused(T *__restrict__ pool
    T *rec = pool + (int64_t) v0 * L.var_stride + (int64_t) slot * L.rec;
    stencil_block<T, ST, NC, NP, V, kThreads, NZ, kDepth, true>(
       rec, L, nv, (A *) smem, red, partials ? partials + (int64_t) v0 * part_stride + in : nullptr,
-      part_stride, &Lo, slot, redir + 6 * (int64_t) in);
+      part_stride, &Lo, slot, redir + 6 * (int64_t) in, edges + (int64_t) kEdgeSrcs * slot);
 }
 
 // Per-block interior sums of variables [var0, var0 + gridDim.y).
@@ -264,7 +266,8 @@ int launch_fused_v(amr_dev *d, int start, int num, double *parts)
    size_t lds = sizeof(typename Acc<T>::type) * 3 * V * (size_t) plane;
 #define GO(NC, NP, NZ)                                                                        \
    hipLaunchKernelGGL((k_stencil_fused<T, ST, NC, NP, V, NZ>), grid, dim3(kThreads), lds,        \
-                      d->stream, (T *) d->pool, d->d_sorted, d->d_redir, L, Lo, start,          \
+                      d->stream, (T *) d->pool, d->d_sorted, d->d_redir, d->d_edges, L, Lo,     \
+                      start,                                                                    \
                       start + num, parts, d->c.max_blocks, xcd_remap_on())
    if (ncol <= kThreads && plane <= 2 * kThreads) {
       if (L.nz == 16) GO(1, 2, 16);

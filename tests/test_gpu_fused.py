@@ -8,7 +8,10 @@ each record.  Every checksum and every final cell (ghosts included) must
 be bit-identical to the plain stages' run, on every golden mesh (7- and
 27-point, permuted exchanges -- where the fused stage is the plain one --,
 variable groups, refinement with stale ghosts), and the golden records of
-the reference must still hold.  A context also refuses the calls that need
+the reference must still hold.  By default the fused stage also drops the
+27-point edge extensions of the faces it reads in place (the stencil and
+the remaining jobs read those edge cells where the extension would have
+copied them from); the plan that keeps them as jobs is tested too.  A context also refuses the calls that need
 written ghost planes until a plain exchange has run.
 """
 import os
@@ -80,6 +83,20 @@ def test_fused_stages_are_identical_in_narrow_storage(name, extra, prec):
     a = A.run(M.EXE_PATH, g["args"], dump=kind, timeout=110, env=dict(env, MINIAMR_FUSED="0"))
     b = A.run(M.EXE_PATH, g["args"], dump=kind, timeout=110, env=dict(env, **FUSED))
     assert len(a["sums"]) > 0
+    assert_same(a, b)
+
+
+# Without edge folding (MINIAMR_FOLD_EDGES=0) the 27-point edge extensions
+# of the faces read in place stay exchange jobs; that plan must agree too.
+@pytest.mark.parametrize("name", ["t27_a", "t27_cv", "cfg4", "cfg1"])
+def test_fused_stages_without_edge_folding(name):
+    require(M.EXE_PATH, "run __graft_entry__.build()")
+    g = A.golden(name)
+    print("case: fused, extensions as jobs %s" % name, flush=True)
+    kind = "cells" if "cells" in g else "bsums_only"
+    a = A.run(M.EXE_PATH, g["args"], dump=kind, timeout=110, env={"MINIAMR_FUSED": "0"})
+    b = A.run(M.EXE_PATH, g["args"], dump=kind, timeout=110,
+              env=dict(FUSED, MINIAMR_FOLD_EDGES="0"))
     assert_same(a, b)
 
 
