@@ -59,6 +59,9 @@ CFG1 = ("--init_x 4 --init_y 4 --init_z 4 --nx 16 --ny 16 --nz 16 --num_vars 40 
         "--object 2 0 0.15 0.15 0.15 0.02 0.02 0.02 0.1 0.1 0.1 0 0 0 "
         "--object 0 0 0.5 0.5 0.5 0.02 0.02 0.02 0.05 0.05 0.05 0 0 0")
 HBM_PEAK_GBS = 8000.0
+# device timing level of the timed runs (amr_dev_set_timing); None keeps the
+# driver's (1: exchange, stencil, checksum and split/consolidate spans)
+TIMING_LEVEL = None
 PREC_NAME = {0: "f64", 1: "f32", 2: "bf16"}
 PREC_BYTES = {0: 8, 1: 4, 2: 2}
 
@@ -93,6 +96,8 @@ def load_host():
     lib.amr_run_stages.argtypes = [ctypes.c_void_p, ctypes.c_int]
     lib.amr_run_device.restype = ctypes.c_void_p
     lib.amr_run_device.argtypes = [ctypes.c_void_p]
+    lib.amr_dev_set_timing.restype = ctypes.c_int
+    lib.amr_dev_set_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
     return lib
 
 
@@ -168,6 +173,9 @@ def run_workload(lib, precision, schedule, warmup, steps, sums_path=None):
         run = lib.amr_run_create(len(argv), arr)
         if not run:
             raise M.AmrError("amr_run_create: " + (lib.amr_run_last_error() or b"").decode())
+        if TIMING_LEVEL is not None and lib.amr_dev_set_timing(lib.amr_run_device(run),
+                                                               TIMING_LEVEL):
+            raise M.AmrError("amr_dev_set_timing failed")
         setup_s += time.time() - t_setup
         ts, rc = 0, 0
         while rc == 0:
@@ -469,7 +477,10 @@ def main():
     ap.add_argument("--no-adjoint-report", action="store_true")
     ap.add_argument("--no-batch-report", action="store_true")
     ap.add_argument("--no-sequential", action="store_true")
+    ap.add_argument("--timing-level", type=int, default=None, choices=[0, 1, 2, 3])
     a = ap.parse_args()
+    global TIMING_LEVEL
+    TIMING_LEVEL = a.timing_level
     if a.gpus != 1:
         print("bench.py: this build runs one rank on one GPU; --gpus %d is not supported "
               "(blocks would shard across GPUs as the reference's ranks do; not built)" % a.gpus,

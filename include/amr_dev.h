@@ -279,7 +279,8 @@ int amr_dev_sync(amr_dev *dev);
  * milliseconds, split as [0] exchange, [1] stencil, [2] checksum,
  * [3] split/consolidate.  Requires amr_dev_set_timing(dev, 1); level 2
  * also times each exchange direction (amr_dev_read_exchange_timing), at
- * the cost of two more events per direction and stage. */
+ * the cost of two more events per direction and stage; level 3 times the
+ * stencil launches only (two events per stage; the other entries read 0). */
 int amr_dev_set_timing(amr_dev *dev, int on);
 int amr_dev_read_timing(amr_dev *dev, double ms[4]);
 /* The exchange time of ms[0] split by direction (x, y, z), accumulated by

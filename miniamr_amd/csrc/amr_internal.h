@@ -186,6 +186,8 @@ struct amr_dev {
 
    bool timing = false;
    bool timing_dirs = false;        // exchange spans per direction (timing level 2)
+   bool timing_stencil_only = false;   // stencil spans only (timing level 3)
+   bool span_skipped = false;       // the last amr_span_begin recorded nothing
    double exchange_ms[3] = {0.0, 0.0, 0.0};   // per direction, read by read_timing
    std::vector<hipEvent_t> ev_free;
    struct Span { hipEvent_t a, b; int cat; };

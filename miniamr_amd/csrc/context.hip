@@ -41,7 +41,8 @@ static hipEvent_t get_event(amr_dev *d)
 
 void amr_span_begin(amr_dev *d, int cat)
 {
-   if (!d->timing)
+   d->span_skipped = !d->timing || (d->timing_stencil_only && cat != 1);
+   if (d->span_skipped)
       return;
    amr_dev::Span s;
    s.a = get_event(d);
@@ -54,7 +55,7 @@ void amr_span_begin(amr_dev *d, int cat)
 
 void amr_span_end(amr_dev *d)
 {
-   if (!d->timing || d->spans.empty())
+   if (d->span_skipped || d->spans.empty())
       return;
    amr_dev::Span &s = d->spans.back();
    s.b = get_event(d);
@@ -935,7 +936,8 @@ int amr_dev_set_timing(amr_dev *d, int on)
    if (!d)
       return AMR_ERR_ARG;
    d->timing = on != 0;
-   d->timing_dirs = on >= 2;
+   d->timing_dirs = on == 2;
+   d->timing_stencil_only = on == 3;
    return AMR_OK;
 }
 

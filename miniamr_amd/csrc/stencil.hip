@@ -27,6 +27,7 @@
 #include <string.h>
 
 #include "kern_stencil.h"
+#include "kern_stencil_vec.h"
 
 namespace {
 
@@ -86,6 +87,33 @@ __global__ __launch_bounds__(kThreads) void k_stencil_fused(T *__restrict__ pool
    stencil_block<T, ST, NC, NP, V, kThreads, NZ, kDepth, true>(
       rec, L, nv, (A *) smem, red, partials ? partials + (int64_t) v0 * part_stride + in : nullptr,
       part_stride, &Lo, slot, redir + 6 * (int64_t) in, edges + (int64_t) kEdgeSrcs * slot);
+}
+
+// The 27-point stencil of blocks with 16 x 16 faces, C columns per thread
+// (kern_stencil_vec.h), plain (interiors read and written in the copy L
+// describes) or fused (as k_stencil_fused).
+template <typename T, bool FUSED>
+__global__ __launch_bounds__(kThreads) void k_stencil_v16(T *__restrict__ pool,
+                                                           const int32_t *__restrict__ sorted,
+                                                           const int32_t *__restrict__ redir,
+                                                           const EdgeSrc *__restrict__ edges,
+                                                           Lay L, Lay Lo, int var0, int var_end,
+                                                           double *__restrict__ partials,
+                                                           int part_stride)
+{
+   typedef typename Acc<T>::type A;
+   constexpr int V = VecVars<T>::v;
+   extern __shared__ __align__(16) unsigned char smem[];
+   __shared__ double red[kThreads / 64];
+   const int in = blockIdx.x;
+   const int v0 = var0 + blockIdx.y * V;
+   const int nv = min(V, var_end - v0);
+   const int slot = sorted[in];
+   T *rec = pool + (int64_t) v0 * L.var_stride + (int64_t) slot * L.rec;
+   stencil27_v16<T, kDepth, FUSED>(
+      rec, L, nv, (A *) smem, red, partials ? partials + (int64_t) v0 * part_stride + in : nullptr,
+      part_stride, FUSED ? &Lo : nullptr, slot, FUSED ? redir + 6 * (int64_t) in : nullptr,
+      FUSED ? edges + (int64_t) kEdgeSrcs * slot : nullptr);
 }
 
 // Per-block interior sums of variables [var0, var0 + gridDim.y).
@@ -199,6 +227,42 @@ static int depth6()
    return v;
 }
 
+// The vector form (k_stencil_v16) serves 27-point stencils of 16 x 16
+// faces whose interiors and x faces start on 16-byte boundaries, unless
+// MINIAMR_STENCIL_VEC=0.  Plain and fused stages then both use it, so their
+// checksum partials are summed in the same order.
+static bool stencil_vec_on()
+{
+   static int v = -1;
+   if (v < 0) {
+      const char *e = getenv("MINIAMR_STENCIL_VEC");
+      v = e ? atoi(e) != 0 : 1;
+   }
+   return v != 0;
+}
+
+template <typename T>
+static bool use_vec(const amr_dev *d, int st)
+{
+   const int C = VecCols<T>::c;
+   const Lay &g = d->g;
+   return st == 27 && g.ny == 16 && g.nz == 16 && stencil_vec_on() && d->oI_par[0] % C == 0 &&
+          d->oI_par[1] % C == 0 && g.oF[0] % C == 0 && g.oF[1] % C == 0 && g.rec % C == 0;
+}
+
+template <typename T, bool FUSED>
+int launch_v16(amr_dev *d, const Lay &L, const Lay &Lo, int start, int num, double *parts)
+{
+   constexpr int V = VecVars<T>::v;
+   dim3 grid(d->n_active, (num + V - 1) / V);
+   const size_t lds = sizeof(typename Acc<T>::type) * 3 * V * 18 * 18;
+   hipLaunchKernelGGL((k_stencil_v16<T, FUSED>), grid, dim3(kThreads), lds, d->stream,
+                      (T *) d->pool, d->d_sorted, d->d_redir, d->d_edges, L, Lo, start,
+                      start + num, parts, d->c.max_blocks);
+   HIP_TRY(d, hipGetLastError());
+   return AMR_OK;
+}
+
 template <typename T, int ST, int V>
 int launch_st_v(amr_dev *d, int start, int num, double *parts)
 {
@@ -237,6 +301,12 @@ int launch_st_v(amr_dev *d, int start, int num, double *parts)
 template <typename T, int ST>
 int launch_st(amr_dev *d, int start, int num, double *parts)
 {
+   if (use_vec<T>(d, ST)) {
+      Lay L;
+      if (int rc = amr_lay(d, start, num, &L))
+         return rc;
+      return launch_v16<T, false>(d, L, L, start, num, parts);
+   }
    if (stencil_vpw<T>() == 4)
       return launch_st_v<T, ST, 4>(d, start, num, parts);
    if (stencil_vpw<T>() == 1)
@@ -293,6 +363,16 @@ int launch_fused_v(amr_dev *d, int start, int num, double *parts)
 template <typename T, int ST>
 int launch_fused(amr_dev *d, int start, int num, double *parts)
 {
+   if (use_vec<T>(d, ST)) {
+      Lay L;
+      if (int rc = amr_lay(d, start, num, &L))
+         return rc;
+      if (int rc = launch_v16<T, true>(d, L, amr_lay_other(d, L), start, num, parts))
+         return rc;
+      for (int v = start; v < start + num; v++)
+         d->par[v] ^= 1;   // the variables' cells now live in the other copy
+      return AMR_OK;
+   }
    if (stencil_vpw<T>() == 4)
       return launch_fused_v<T, ST, 4>(d, start, num, parts);
    if (stencil_vpw<T>() == 1)
