@@ -290,6 +290,14 @@ struct EdgeResolver {
    const Lay &g;
    const std::vector<int32_t> &same;
    const std::vector<uint8_t> &bc;
+   Lay L0;   // g with the interior copies at offset 0 (ES_I / ES_ZI offsets)
+
+   EdgeResolver(const Lay &g_, const std::vector<int32_t> &same_, const std::vector<uint8_t> &bc_)
+      : g(g_), same(same_), bc(bc_), L0(g_)
+   {
+      L0.oI = 0;
+      L0.oZI = 0;
+   }
 
    // the cell whose value block b's cell (i, j, k) holds when the stencil runs
    Cell resolve(int b, int i, int j, int k) const
@@ -327,9 +335,6 @@ struct EdgeResolver {
       const bool ghost = c.i == 0 || c.i == g.nx + 1 || c.j == 0 || c.j == g.ny + 1 ||
                          c.k == 0 || c.k == g.nz + 1;
       const bool zc = axis != 2 && (c.k == 1 || c.k == g.nz);
-      Lay L0 = g;
-      L0.oI = 0;
-      L0.oZI = 0;
       e.slot = c.slot;
       e.stride = 0;
       if (!ghost) {
@@ -344,7 +349,10 @@ struct EdgeResolver {
 
    // entry of a line of n cells, cell(t) giving block b's cell t (resolved
    // when folding, else the block's own); false if the cells are not one
-   // affine run of one block and kind
+   // affine run of one block and kind.  Every cell of a line follows the
+   // same copies (the jobs that write a line, and the in-place faces on the
+   // way, are those of its whole face), so the first two cells give the
+   // run and the last one checks it.
    template <typename F>
    bool line(int b, int axis, int n, F cell, bool fold, EdgeSrc &e) const
    {
@@ -355,16 +363,13 @@ struct EdgeResolver {
       if (!at(0, e))
          return false;
       if (n > 1) {
-         EdgeSrc e1;
+         EdgeSrc e1, el;
          if (!at(1, e1) || e1.slot != e.slot || e1.kind != e.kind)
             return false;
          e.stride = e1.off - e.off;
-         for (int t = 2; t < n; t++) {
-            EdgeSrc et;
-            if (!at(t, et) || et.slot != e.slot || et.kind != e.kind ||
-                et.off != e.off + t * e.stride)
-               return false;
-         }
+         if (n > 2 && (!at(n - 1, el) || el.slot != e.slot || el.kind != e.kind ||
+                       el.off != e.off + (n - 1) * e.stride))
+            return false;
       }
       return true;
    }
@@ -430,7 +435,7 @@ static int build_fused_plan(amr_dev *d, const int32_t *sorted, int n_active,
    std::vector<EdgeSrc> edges(kEdgeSrcs * (size_t) M, EdgeSrc{0, 0, 0, ES_ABS});
    bool fold = d->g.stencil == 27 && fold_edges_on();
    {
-      EdgeResolver R{d->g, inp, bc};
+      const EdgeResolver R(d->g, inp, bc);
       for (int in = 0; in < n_active && fold; in++)
          fold = R.block(sorted[in], true, &edges[kEdgeSrcs * (size_t) sorted[in]]);
       if (!fold)

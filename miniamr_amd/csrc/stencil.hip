@@ -92,7 +92,7 @@ __global__ __launch_bounds__(kThreads) void k_stencil_fused(T *__restrict__ pool
 // The 27-point stencil of blocks with 16 x 16 faces, C columns per thread
 // (kern_stencil_vec.h), plain (interiors read and written in the copy L
 // describes) or fused (as k_stencil_fused).
-template <typename T, bool FUSED>
+template <typename T, bool FUSED, int D>
 __global__ __launch_bounds__(kThreads) void k_stencil_v16(T *__restrict__ pool,
                                                            const int32_t *__restrict__ sorted,
                                                            const int32_t *__restrict__ redir,
@@ -110,7 +110,7 @@ __global__ __launch_bounds__(kThreads) void k_stencil_v16(T *__restrict__ pool,
    const int nv = min(V, var_end - v0);
    const int slot = sorted[in];
    T *rec = pool + (int64_t) v0 * L.var_stride + (int64_t) slot * L.rec;
-   stencil27_v16<T, kDepth, FUSED>(
+   stencil27_v16<T, D, FUSED>(
       rec, L, nv, (A *) smem, red, partials ? partials + (int64_t) v0 * part_stride + in : nullptr,
       part_stride, FUSED ? &Lo : nullptr, slot, FUSED ? redir + 6 * (int64_t) in : nullptr,
       FUSED ? edges + (int64_t) kEdgeSrcs * slot : nullptr);
@@ -250,14 +250,22 @@ static bool use_vec(const amr_dev *d, int st)
           d->oI_par[1] % C == 0 && g.oF[0] % C == 0 && g.oF[1] % C == 0 && g.rec % C == 0;
 }
 
+// Planes of loads in flight per workgroup of the vector form.  Measured on
+// the 16^3 workload, ms per fused launch for 2 / 4 / 6 / 8 planes: FP64
+// 2.853 / 2.844 / 2.894 / 2.894, FP32 1.614 / 1.599 / 1.865 / 1.825, BF16
+// 1.065 / 1.192 / 1.440 / 1.191.  More planes cost registers, hence
+// workgroups per CU (BF16 at 2 planes: 64 VGPRs, 7 waves per SIMD; at 4:
+// 82, 5); the loads in flight per CU stay about the same.
+template <typename T> struct VecDepth { static constexpr int d = sizeof(T) == 2 ? 2 : 4; };
+
 template <typename T, bool FUSED>
 int launch_v16(amr_dev *d, const Lay &L, const Lay &Lo, int start, int num, double *parts)
 {
    constexpr int V = VecVars<T>::v;
    dim3 grid(d->n_active, (num + V - 1) / V);
    const size_t lds = sizeof(typename Acc<T>::type) * 3 * V * 18 * 18;
-   hipLaunchKernelGGL((k_stencil_v16<T, FUSED>), grid, dim3(kThreads), lds, d->stream,
-                      (T *) d->pool, d->d_sorted, d->d_redir, d->d_edges, L, Lo, start,
+   hipLaunchKernelGGL((k_stencil_v16<T, FUSED, VecDepth<T>::d>), grid, dim3(kThreads), lds,
+                      d->stream, (T *) d->pool, d->d_sorted, d->d_redir, d->d_edges, L, Lo, start,
                       start + num, parts, d->c.max_blocks);
    HIP_TRY(d, hipGetLastError());
    return AMR_OK;
