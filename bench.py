@@ -145,6 +145,18 @@ def cpu_baseline(per_ts, timeout=300):
                                    "speedup": round(u / s / (updates / t), 1) if s > 0 else None}}
 
 
+def copy_ceiling(bytes_=4 << 30, reps=3):
+    """The streaming copy and read bandwidth of this GPU in GB/s
+    (amr_dev_copy_bandwidth: one 16-byte nontemporal access per lane, one
+    workgroup per 4 KiB tile), or None where it cannot run."""
+    dev = M.load_library()
+    c, r = ctypes.c_double(), ctypes.c_double()
+    if dev.amr_dev_copy_bandwidth(bytes_, reps, ctypes.byref(c), ctypes.byref(r)):
+        return None
+    return {"copy_gbs": round(c.value, 1), "read_gbs": round(r.value, 1),
+            "buffer_bytes": bytes_}
+
+
 def run_workload(lib, precision, schedule, warmup, steps, sums_path=None):
     """Timesteps warmup+1 .. warmup+steps of the workload, timed.
 
@@ -517,6 +529,7 @@ def main():
             block_vars = acc["block_stages"] * s1.num_vars / launches
             traffic = round(per_bv * block_vars)
     avg_blocks = acc["block_stages"] / max(1, acc["stages"])
+    ceiling = copy_ceiling()
     timed_ts = sorted(r["per_ts"])
     same = [r["per_ts"][k] for k in timed_ts]
     gpu_sample = {"timesteps": timed_ts,
@@ -582,10 +595,18 @@ def main():
                      "traffic_frac": (round(traffic / (stencil_s / launches) / 1e9 /
                                             HBM_PEAK_GBS, 4)
                                       if traffic and launches and stencil_s > 0 else None),
-                     "kernel": "k_stencil_fused + k_stencil (27-point; every stage but "
+                     "kernel": "k_stencil_v16 (27-point, 16x16 faces; every stage but "
                                "a timestep's last reads same-level faces in place), %d "
                                "launches, %.3f ms avg" % (
                                    launches, stencil_s / max(1, launches) * 1e3),
+                     # the practical ceiling on this GPU: a streaming copy
+                     # and read (amr_dev_copy_bandwidth, 4 GiB, best of 3),
+                     # and the stencil's PMC-measured bytes against it
+                     "copy_ceiling": ceiling,
+                     "traffic_frac_of_copy": (round(traffic / (stencil_s / launches) / 1e9 /
+                                                    ceiling["copy_gbs"], 4)
+                                              if traffic and launches and stencil_s > 0
+                                              and ceiling else None),
                      "step_level_frac": round(updates * 2 * ebytes / elapsed / 1e9 /
                                               HBM_PEAK_GBS, 4)},
         "device_ms": {"exchange": round(acc["dev"][0], 2),

@@ -275,6 +275,15 @@ int amr_dev_stage_fused(amr_dev *dev, int start, int num, int stage, int permute
 /* Wait for all enqueued work. */
 int amr_dev_sync(amr_dev *dev);
 
+/* The bandwidth a plain streaming kernel reaches on the current device, in
+ * GB/s: a copy of `bytes` bytes (a multiple of 4096; one 16-byte
+ * nontemporal load and store per lane, one workgroup of 256 threads per
+ * 4 KiB tile) and a read of the same buffer, best of `reps` launches each.
+ * Allocates and frees two buffers of `bytes`.  No counterpart in the
+ * reference: it measures the practical ceiling that the stencil's roofline
+ * figure is read against (bench.py reports both). */
+int amr_dev_copy_bandwidth(size_t bytes, int reps, double *copy_gbs, double *read_gbs);
+
 /* Device time of the kernels enqueued since the last call, in
  * milliseconds, split as [0] exchange, [1] stencil, [2] checksum,
  * [3] split/consolidate.  Requires amr_dev_set_timing(dev, 1); level 2

@@ -39,7 +39,7 @@ ABI_SYMBOLS = [
     "amr_dev_stage_fused", "amr_dev_set_materials", "amr_dev_stencil_stage",
     "amr_dev_read_check_counts", "amr_dev_mark_refine", "amr_dev_split_noalloc",
     "amr_dev_face_msg_len", "amr_dev_pack_faces", "amr_dev_unpack_faces",
-    "amr_dev_compare_cells",
+    "amr_dev_compare_cells", "amr_dev_copy_bandwidth",
 ]
 
 
@@ -105,6 +105,7 @@ def load_library(path=LIB_PATH):
         "amr_dev_split_noalloc": (ip, [vp, vp, ip]),
         "amr_dev_face_msg_len": (ip, [vp, ip, ip, ip, ip]),
         "amr_dev_compare_cells": (ip, [vp, vp, ip, ip, dp]),
+        "amr_dev_copy_bandwidth": (ip, [ctypes.c_size_t, ip, dp, dp]),
         "amr_dev_pack_faces": (ip, [vp, ip, ip, ip, ip, vp, ip, vp, ctypes.c_size_t]),
         "amr_dev_unpack_faces": (ip, [vp, ip, ip, ip, ip, vp, ip, vp, ctypes.c_size_t]),
         "amr_dev_consolidate": (ip, [vp, vp, ip]),

@@ -984,6 +984,16 @@ int amr_dev_unpack_faces(amr_dev *d, int code, int dir, int start, int num_comm,
    return rc;
 }
 
+/* The device's streaming bandwidth has no CPU meaning: no device here. */
+int amr_dev_copy_bandwidth(size_t bytes, int reps, double *copy_gbs, double *read_gbs)
+{
+   (void) bytes;
+   (void) reps;
+   (void) copy_gbs;
+   (void) read_gbs;
+   return AMR_ERR_NODEV;
+}
+
 /* Largest |a - b| over the active blocks' interior cells, per variable
  * (FP64 contexts only in this CPU library). */
 int amr_dev_compare_cells(amr_dev *a, amr_dev *b, int start, int num, double *max_abs)

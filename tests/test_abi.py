@@ -46,7 +46,7 @@ def host_symbols():
 
 
 def test_headers_declare_what_is_expected():
-    assert len(device_symbols()) == 29 + 13
+    assert len(device_symbols()) == 30 + 13
     assert len(adjoint_symbols()) == 11 + 10
     assert "amr_batch_timestep" in device_symbols()
     assert "amr_run_step" in host_symbols() and "amr_batch_run_step" in host_symbols()
