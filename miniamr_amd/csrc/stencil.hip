@@ -250,14 +250,6 @@ static bool use_vec(const amr_dev *d, int st)
           d->oI_par[1] % C == 0 && g.oF[0] % C == 0 && g.oF[1] % C == 0 && g.rec % C == 0;
 }
 
-// Planes of loads in flight per workgroup of the vector form.  Measured on
-// the 16^3 workload, ms per fused launch for 2 / 4 / 6 / 8 planes: FP64
-// 2.853 / 2.844 / 2.894 / 2.894, FP32 1.614 / 1.599 / 1.865 / 1.825, BF16
-// 1.065 / 1.192 / 1.440 / 1.191.  More planes cost registers, hence
-// workgroups per CU (BF16 at 2 planes: 64 VGPRs, 7 waves per SIMD; at 4:
-// 82, 5); the loads in flight per CU stay about the same.
-template <typename T> struct VecDepth { static constexpr int d = sizeof(T) == 2 ? 2 : 4; };
-
 template <typename T, bool FUSED>
 int launch_v16(amr_dev *d, const Lay &L, const Lay &Lo, int start, int num, double *parts)
 {
