@@ -257,6 +257,19 @@ Lay amr_lay_other(const amr_dev *d, const Lay &L)
    return o;
 }
 
+// MINIAMR_HOIST=0 keeps every fused job in its own direction's launch (the
+// default moves the y and z coarse/fine jobs of blocks without a boundary
+// face into the first launch, below).
+static bool hoist_on()
+{
+   static int v = -1;
+   if (v < 0) {
+      const char *e = getenv("MINIAMR_HOIST");
+      v = e ? atoi(e) != 0 : 1;
+   }
+   return v != 0;
+}
+
 // MINIAMR_FOLD_EDGES=0 keeps the 27-point edge extensions of in-place faces
 // as exchange jobs (the default folds them into the reads, below).
 static bool fold_edges_on()
@@ -443,9 +456,14 @@ static int build_fused_plan(amr_dev *d, const int32_t *sorted, int n_active,
             R.block(sorted[in], false, &edges[kEdgeSrcs * (size_t) sorted[in]]);
    }
    d->fold_edges = fold;
-   std::vector<FusedJob> fj;
-   fj.reserve(nj);
-   int32_t fc[3] = {0, 0, 0};
+   // Jobs by launch.  The y and z prolongations and restrictions of blocks
+   // without a boundary face go with the x phase: they read only interior
+   // cells, which no job writes, and write ghost cells that only later
+   // phases' jobs (27-point rings) and the stencil read -- an x job reads
+   // ghost cells only in a boundary fill of its own block.  The y and z
+   // launches keep the jobs that read rings.
+   std::vector<FusedJob> ph[3];
+   const bool hoist = hoist_on();
    for (int t = 0, dir = 0; dir < 3; dir++)
       for (int e = 0; e < count[dir]; e++, t++) {
          const amr_face_job &j = jobs[t];
@@ -469,9 +487,16 @@ static int build_fused_plan(amr_dev *d, const int32_t *sorted, int n_active,
                f.alt[3] = inplace(j.src, 3);
             }
          }
-         fj.push_back(f);
-         fc[dir]++;
+         const bool cf = j.kind == AMR_FACE_PROLONG || j.kind == AMR_FACE_RESTRICT;
+         ph[hoist && dir > 0 && cf && !bc[j.dst] ? 0 : dir].push_back(f);
       }
+   std::vector<FusedJob> fj;
+   fj.reserve(nj);
+   int32_t fc[3];
+   for (int p = 0; p < 3; p++) {
+      fj.insert(fj.end(), ph[p].begin(), ph[p].end());
+      fc[p] = (int32_t) ph[p].size();
+   }
    std::vector<int32_t> redir(6 * (size_t) std::max(n_active, 1), -1);
    for (int in = 0; in < n_active; in++)
       for (int f = 0; f < 6; f++)

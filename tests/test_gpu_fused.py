@@ -103,8 +103,8 @@ def test_fused_stages_without_edge_folding(name):
 # Meshes of 16 x 16-face blocks, where the stencil is the vector form:
 # refinement around moving objects gives coarse/fine faces in all three
 # directions, boundary blocks and stale ghosts.  Fused and plain stages must
-# agree bit for bit, cells and ghosts included, in every precision and with
-# edge folding switched off.
+# agree bit for bit, cells and ghosts included, in every precision, with
+# edge folding switched off and with every job in its own direction's launch.
 VEC_MESHES = [
     "--nx 16 --ny 16 --nz 16 --init_x 2 --init_y 2 --init_z 2 --num_vars 6 --stencil 27 "
     "--num_tsteps 4 --stages_per_ts 6 --checksum_freq 2 --num_refine 2 --refine_freq 2 "
@@ -117,10 +117,11 @@ VEC_MESHES = [
     "--object 0 0 0.2 0.7 0.3 0.03 -0.02 0.04 0.1 0.1 0.1 0 0 0",
 ]
 VEC_ENVS = [{}, {"MINIAMR_PRECISION": "f32"}, {"MINIAMR_PRECISION": "bf16"},
-            {"MINIAMR_FOLD_EDGES": "0"}]
+            {"MINIAMR_FOLD_EDGES": "0"}, {"MINIAMR_HOIST": "0"}]
 
 
-@pytest.mark.parametrize("env", VEC_ENVS, ids=["f64", "f32", "bf16", "no-edge-fold"])
+@pytest.mark.parametrize("env", VEC_ENVS,
+                         ids=["f64", "f32", "bf16", "no-edge-fold", "no-hoist"])
 @pytest.mark.parametrize("mesh", range(len(VEC_MESHES)))
 def test_fused_stages_vector_stencil(mesh, env):
     require(M.EXE_PATH, "run __graft_entry__.build()")
