@@ -254,21 +254,23 @@ int amr_dev_stage(amr_dev *dev, int start, int num, int stage, int permute,
                   int sum_row);
 
 /* amr_dev_stage with same-level faces read in place (no counterpart in the
- * reference's interface; the drop-in driver uses it for every stage of a
- * timestep but the last).  Cells, checksums and every ghost value a stencil
- * reads are those of amr_dev_stage.  The difference is where they come
- * from: the ghost planes of same-level faces of blocks away from the
- * domain boundary are not written -- the stencil reads those cells from
- * the neighbour's interior, which stays unchanged because the stage writes
- * its results to a second copy of every interior (records hold two
- * unless the context was created with MINIAMR_FUSED=0 in the environment,
- * which costs ~70% more pool memory at 16^3).  Those
- * ghost planes are stale until the next amr_dev_stage or amr_dev_comm of
- * the same variables; until then set_mesh, split, consolidate, stencil,
- * download_block and save_state fail with AMR_ERR_STATE.  It is
- * amr_dev_stage without two interiors, with permute != 0, while the tape
- * records, and for stages of fewer than 1e8 cell updates (latency-bound;
- * MINIAMR_FUSED=2 at creation fuses those too). */
+ * reference's interface; the drop-in driver uses it for every stage but the
+ * last one of a timestep that ends in a refinement or ends the run).
+ * Cells, checksums and every ghost value a stencil reads are those of
+ * amr_dev_stage.  The difference is where they come from: the ghost planes
+ * of same-level faces of blocks away from the domain boundary, and their
+ * 27-point edge extensions, are not written -- the stencil reads those
+ * cells from the neighbours' interiors, which stay unchanged because the
+ * stage writes its results to a second copy of every interior (records
+ * hold two unless the context was created with MINIAMR_FUSED=0 in the
+ * environment, which costs ~70% more pool memory at 16^3).  Those ghost
+ * cells are stale until the next amr_dev_stage or amr_dev_comm of the same
+ * variables; set_mesh, split, consolidate, stencil, download_block, the
+ * halo codec and save_state first bring them up to date (to the values
+ * amr_dev_stage would have written, from the interiors the stage read).
+ * It is amr_dev_stage without two interiors, with permute != 0, while the
+ * tape records, and for stages of fewer than 1e8 cell updates
+ * (latency-bound; MINIAMR_FUSED=2 at creation fuses those too). */
 int amr_dev_stage_fused(amr_dev *dev, int start, int num, int stage, int permute,
                         int sum_row);
 

@@ -173,6 +173,13 @@ struct amr_dev {
    FusedJob *d_fjobs = nullptr;
    int32_t fcount[3] = {0, 0, 0};
    int fjobs_cap = 0;
+   // the plain jobs of the faces a fused stage reads in place (same-level
+   // jobs of blocks without a boundary face), per direction: rerun on the
+   // interiors the last fused stage read, they write the ghost cells that
+   // stage left unwritten (amr_require_fresh)
+   amr_face_job *d_mjobs = nullptr;
+   int32_t mcount[3] = {0, 0, 0};
+   int mjobs_cap = 0;
    int32_t *d_redir = nullptr;
    // kEdgeSrcs entries per slot (EdgeSrc); fold_edges: the plan drops the
    // 27-point edge extensions of in-place faces (the stencil and the
@@ -368,9 +375,14 @@ int amr_launch_mat_stencil(amr_dev *d, int var, int calc_stage);
 // to the other one (the variables' copy flips), same-level faces read in
 // place.
 int amr_launch_stencil_fused(amr_dev *d, int start, int num, bool want_partials);
-// AMR_ERR_STATE when any of variables [start, start+num) has stale ghost
-// planes (an amr_dev_stage_fused since its last plain exchange).
+// Brings the ghost planes of variables [start, start+num) that an
+// amr_dev_stage_fused left unwritten (stale) up to date: they are set to what
+// that stage's plain exchange would have written.
 int amr_require_fresh(amr_dev *d, int start, int num, const char *what);
+// Plain exchange jobs jobs[0..n) of direction-ordered phases over variables
+// [start, start+num), reading and writing the copy L describes (exchange.hip).
+int amr_launch_exchange_jobs(amr_dev *d, const amr_face_job *jobs, int n, const Lay &L,
+                             int start, int num);
 int amr_launch_block_sums(amr_dev *d, int start, int num);
 int amr_launch_reduce_sums(amr_dev *d, int start, int num, double *dst_row);
 int amr_launch_split(amr_dev *d, const amr_split_job *d_jobs, int n, void *scratch);

@@ -179,6 +179,7 @@ int amr_dev_destroy(amr_dev *d)
    hipFree(d->d_fjobs);
    hipFree(d->d_redir);
    hipFree(d->d_edges);
+   hipFree(d->d_mjobs);
    hipFree(d->d_a0);
    hipFree(d->d_check_counts);
    hipFree(d->d_matwork);
@@ -490,6 +491,15 @@ static int build_fused_plan(amr_dev *d, const int32_t *sorted, int n_active,
          const bool cf = j.kind == AMR_FACE_PROLONG || j.kind == AMR_FACE_RESTRICT;
          ph[hoist && dir > 0 && cf && !bc[j.dst] ? 0 : dir].push_back(f);
       }
+   // the jobs that bring stale ghost planes up to date (amr_require_fresh)
+   std::vector<amr_face_job> mj;
+   int32_t mc[3] = {0, 0, 0};
+   for (int t = 0, dir = 0; dir < 3; dir++)
+      for (int e = 0; e < count[dir]; e++, t++)
+         if (jobs[t].kind == AMR_FACE_SAME && inplace(jobs[t].dst, jobs[t].face) >= 0) {
+            mj.push_back(jobs[t]);
+            mc[dir]++;
+         }
    std::vector<FusedJob> fj;
    fj.reserve(nj);
    int32_t fc[3];
@@ -521,17 +531,55 @@ static int build_fused_plan(amr_dev *d, const int32_t *sorted, int n_active,
                         hipMemcpyHostToDevice));
    for (int i = 0; i < 3; i++)
       d->fcount[i] = fc[i];
+   if ((int) mj.size() > d->mjobs_cap) {
+      const int cap = std::max((int) mj.size(), 2 * d->mjobs_cap);
+      hipFree(d->d_mjobs);
+      d->d_mjobs = nullptr;
+      d->mjobs_cap = 0;
+      HIP_TRY(d, hipMalloc(&d->d_mjobs, sizeof(amr_face_job) * cap));
+      d->mjobs_cap = cap;
+   }
+   if (!mj.empty())
+      HIP_TRY(d, hipMemcpy(d->d_mjobs, mj.data(), sizeof(amr_face_job) * mj.size(),
+                           hipMemcpyHostToDevice));
+   for (int i = 0; i < 3; i++)
+      d->mcount[i] = mc[i];
    return AMR_OK;
 }
 
+// A fused stage leaves unwritten the ghost cells its stencil reads where
+// they are: the planes of the faces read in place and their 27-point edge
+// extensions, which are exactly what the same-level jobs of blocks without
+// a boundary face write (every other job runs in the fused stage).  The
+// interiors those jobs read in the plain stage are the ones the fused stage
+// read, which the record still holds in its other interior copy until the
+// next stage writes it; a job's reads of other ghost cells find them as the
+// plain stage's earlier phases left them (written by the fused stage, or by
+// this rerun's earlier directions).  So rerunning those jobs, in direction
+// order, on the other copy gives every ghost cell the plain stage's value.
 int amr_require_fresh(amr_dev *d, int start, int num, const char *what)
 {
-   for (int v = start; v < start + num && v < (int) d->stale.size(); v++)
-      if (d->stale[v])
-         return amr_set_error(d, AMR_ERR_STATE,
-                              "%s: variable %d has ghost planes read in place by "
-                              "amr_dev_stage_fused; run amr_dev_stage (or amr_dev_comm) on it "
-                              "first", what, v);
+   const int end = std::min(start + num, (int) d->stale.size());
+   for (int a = start; a < end;) {
+      if (!d->stale[a]) {
+         a++;
+         continue;
+      }
+      int b = a + 1;   // a run of stale variables whose cells are in one copy
+      while (b < end && d->stale[b] && d->par[b] == d->par[a])
+         b++;
+      Lay L;
+      if (int rc = amr_lay(d, a, b - a, &L))
+         return rc;
+      const Lay Lr = amr_lay_other(d, L);
+      for (int dir = 0, off = 0; dir < 3; off += d->mcount[dir], dir++)
+         if (int rc = amr_launch_exchange_jobs(d, d->d_mjobs + off, d->mcount[dir], Lr, a, b - a))
+            return amr_set_error(d, rc, "%s: bringing ghost planes up to date: %s", what,
+                                 d->err);
+      for (int v = a; v < b; v++)
+         d->stale[v] = 0;
+      a = b;
+   }
    return AMR_OK;
 }
 

@@ -156,6 +156,32 @@ int amr_launch_exchange_fused_dir(amr_dev *d, int dir, int start, int num)
    }
 }
 
+namespace {
+
+template <typename T>
+int launch_jobs(amr_dev *d, const amr_face_job *jobs, int n, const Lay &L, int start, int num)
+{
+   dim3 grid(n, (num + kVars - 1) / kVars);
+   hipLaunchKernelGGL((k_exchange<T, kVars, 256>), grid, dim3(256), 0, d->stream, (T *) d->pool,
+                      jobs, L, start, start + num);
+   HIP_TRY(d, hipGetLastError());
+   return AMR_OK;
+}
+
+}  // namespace
+
+int amr_launch_exchange_jobs(amr_dev *d, const amr_face_job *jobs, int n, const Lay &L,
+                             int start, int num)
+{
+   if (!n || !num)
+      return AMR_OK;
+   switch (d->c.precision) {
+   case AMR_PREC_F64: return launch_jobs<double>(d, jobs, n, L, start, num);
+   case AMR_PREC_F32: return launch_jobs<float>(d, jobs, n, L, start, num);
+   default: return launch_jobs<uint16_t>(d, jobs, n, L, start, num);
+   }
+}
+
 int amr_launch_exchange_dir(amr_dev *d, int dir, int start, int num)
 {
    if (!d->d_jobs && (d->count[0] + d->count[1] + d->count[2]))

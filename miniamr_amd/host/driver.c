@@ -89,16 +89,37 @@ double amr_calc_time_step(amr_run *R)
    return delta > 0.0 ? 1.0 / delta : 1.0;
 }
 
+/* Whether every ghost plane must hold what the reference's would at the
+ * end of the current timestep: before a refinement (split, consolidate and
+ * the new mesh's plan reuse slots whose ghost cells later stencils read as
+ * they are), at the end of the run (dumps, saved state), and whenever the
+ * run's length is not known in advance (--time).  Between two timesteps on
+ * the same mesh nothing reads a ghost plane that a fused stage leaves
+ * unwritten: the next stages read those faces in place again.
+ * MINIAMR_PLAIN_LAST=1 ends every timestep with a plain stage. */
+static int ghosts_needed_after_timestep(const amr_run *R)
+{
+   const amr_params *P = &R->P;
+   static int every = -1;
+   if (every < 0) {
+      const char *e = getenv("MINIAMR_PLAIN_LAST");
+      every = e && atoi(e) != 0;
+   }
+   const int ts = R->loop.ts;
+   return every || P->use_time || ts >= P->num_tsteps ||
+          (P->num_refine && !P->uniform_refine && !(ts % P->refine_freq));
+}
+
 /* One stage of variables [start, start+number).  Every stage of a
  * timestep but the last reads same-level faces in place (amr_dev_stage_fused:
- * the same cells and checksums); the last is a plain stage, so that at the
- * end of every timestep -- where refinement, checksums and dumps look at
- * the cells -- every ghost plane holds what the reference's would. */
+ * the same cells and checksums); the last is a plain stage where the ghost
+ * planes are looked at after it (ghosts_needed_after_timestep), so that
+ * there every ghost plane holds what the reference's would. */
 static void enqueue_stage(amr_run *R, int start, int number, int stage, int comm_stage,
                           int row)
 {
    const amr_params *P = &R->P;
-   if (stage < P->stages_per_ts - 1)
+   if (stage < P->stages_per_ts - 1 || !ghosts_needed_after_timestep(R))
       amr_dev_check(R, amr_dev_stage_fused(R->dev, start, number, comm_stage, P->permute, row),
                     "stage");
    else

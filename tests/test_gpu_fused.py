@@ -1,8 +1,9 @@
 """Stages that read same-level faces in place (amr_dev_stage_fused).
 
-The drop-in driver runs every stage of a timestep but the last as a fused
-stage (by default when a stage has >= 1e8 cell updates; MINIAMR_FUSED=2 at
-every size, as here): the stencil reads same-level neighbours'
+The drop-in driver runs every stage of a timestep as a fused stage but the
+last one before a refinement or the end of the run (by default when a stage
+has >= 1e8 cell updates; MINIAMR_FUSED=2 at every size, as here): the
+stencil reads same-level neighbours'
 interiors instead of ghost copies, writing to the second interior copy of
 each record.  Every checksum and every final cell (ghosts included) must
 be bit-identical to the plain stages' run, on every golden mesh (7- and
@@ -11,7 +12,8 @@ variable groups, refinement with stale ghosts), and the golden records of
 the reference must still hold.  By default the fused stage also drops the
 27-point edge extensions of the faces it reads in place (the stencil and
 the remaining jobs read those edge cells where the extension would have
-copied them from); the plan that keeps them as jobs is tested too.  A context also refuses the calls that need
+copied them from); the plan that keeps them as jobs is tested too.  A
+context also refuses the calls that need
 written ghost planes until a plain exchange has run.
 """
 import os
@@ -104,7 +106,9 @@ def test_fused_stages_without_edge_folding(name):
 # refinement around moving objects gives coarse/fine faces in all three
 # directions, boundary blocks and stale ghosts.  Fused and plain stages must
 # agree bit for bit, cells and ghosts included, in every precision, with
-# edge folding switched off and with every job in its own direction's launch.
+# edge folding switched off, with every job in its own direction's launch, and
+# with a plain stage at the end of every timestep (by default only before a
+# refinement and at the end of the run).
 VEC_MESHES = [
     "--nx 16 --ny 16 --nz 16 --init_x 2 --init_y 2 --init_z 2 --num_vars 6 --stencil 27 "
     "--num_tsteps 4 --stages_per_ts 6 --checksum_freq 2 --num_refine 2 --refine_freq 2 "
@@ -117,11 +121,11 @@ VEC_MESHES = [
     "--object 0 0 0.2 0.7 0.3 0.03 -0.02 0.04 0.1 0.1 0.1 0 0 0",
 ]
 VEC_ENVS = [{}, {"MINIAMR_PRECISION": "f32"}, {"MINIAMR_PRECISION": "bf16"},
-            {"MINIAMR_FOLD_EDGES": "0"}, {"MINIAMR_HOIST": "0"}]
+            {"MINIAMR_FOLD_EDGES": "0"}, {"MINIAMR_HOIST": "0"}, {"MINIAMR_PLAIN_LAST": "1"}]
 
 
 @pytest.mark.parametrize("env", VEC_ENVS,
-                         ids=["f64", "f32", "bf16", "no-edge-fold", "no-hoist"])
+                         ids=["f64", "f32", "bf16", "no-edge-fold", "no-hoist", "plain-last"])
 @pytest.mark.parametrize("mesh", range(len(VEC_MESHES)))
 def test_fused_stages_vector_stencil(mesh, env):
     require(M.EXE_PATH, "run __graft_entry__.build()")
@@ -134,22 +138,30 @@ def test_fused_stages_vector_stencil(mesh, env):
     assert_same(a, b)
 
 
-def test_stale_ghosts_are_refused(monkeypatch):
+def test_stale_ghosts_are_brought_up_to_date(monkeypatch):
+    """After fused stages, the calls that look at ghost planes first give
+    them the values the plain stages would have written (from the interiors
+    the last fused stage read): downloads after fused and after plain stages
+    agree bit for bit, ghosts included, and set_mesh accepts the mesh."""
     require(M.LIB_PATH, "run __graft_entry__.build()")
     monkeypatch.setenv("MINIAMR_FUSED", "2")
-    rng = np.random.default_rng(5)
-    d = M.Device(nx=6, ny=4, nz=8, num_vars=3, max_blocks=14, stencil=27)
-    for s in range(11):
-        d.upload(s, rng.random(d.shape))
-    plan = meshgen.random_plan(rng, 5, 6)
-    order = list(range(11))
-    d.set_mesh(order, plan)
-    d.stage_fused(0, 3, stage=0)
-    with pytest.raises(M.AmrError):
-        d.set_mesh(order, plan)
-    with pytest.raises(M.AmrError):
-        d.download(0)
-    d.comm(0, 3, stage=1)          # a plain exchange writes every ghost plane again
-    d.set_mesh(order, plan)
-    d.download(0)
-    d.close()
+    for st in (27, 7):
+        rng = np.random.default_rng(5)
+        cells = [rng.random((3, 8, 6, 10)) for _ in range(11)]
+        plan = meshgen.random_plan(rng, 5, 6)
+        order = list(range(11))
+        devs = []
+        for fused in (True, False):
+            d = M.Device(nx=6, ny=4, nz=8, num_vars=3, max_blocks=14, stencil=st)
+            for s_, c in enumerate(cells):
+                d.upload(s_, c)
+            d.set_mesh(order, plan)
+            for k in range(3):
+                (d.stage_fused if fused else d.stage)(0, 3, stage=k)
+            devs.append(d)
+        for s_ in order:
+            a, b = devs[0].download(s_), devs[1].download(s_)
+            assert np.array_equal(a, b), (st, s_)
+        devs[0].set_mesh(order, plan)
+        for d in devs:
+            d.close()
