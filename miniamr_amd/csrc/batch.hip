@@ -58,9 +58,14 @@ struct BMember {
    const int32_t *sorted;
    double *partials;
    double *sums;
-   Lay L;
+   Lay L;                   // the interior copy holding the cells
+   Lay Lo;                  // the other copy (fused stages write it)
+   const FusedJob *fjobs;   // the member's fused plan (amr_dev_stage_fused)
+   const int32_t *redir;
+   const EdgeSrc *edges;
    int n_active, num_vars, part_stride, cls;
    int count[3], joff[3];
+   int fcount[3], fjoff[3];
 };
 
 // Shape classes of the wave-per-item stencil: columns (ny*nz) per lane and
@@ -106,6 +111,11 @@ struct BStItem {          // stencil: one block, a group of variables
    void *rec;             // the block's record of the first variable
    double *part;          // its checksum partial of the first variable
    int32_t m, nv;         // member, variables in the group
+   int32_t in, slot;      // the block's place in the sorted list, its slot
+};
+struct BFxItem {          // fused-stage exchange: one fused job, a group of variables
+   FusedJob fj;
+   int32_t m, v0;
 };
 
 __global__ void k_mat_exchange(const BMember *__restrict__ mem, const int *__restrict__ pre,
@@ -141,6 +151,25 @@ __global__ void k_mat_stencil(const BMember *__restrict__ mem, const int *__rest
       it.part = b.partials + (int64_t) v0 * b.part_stride + in;
       it.m = m;
       it.nv = min(vpw, b.num_vars - v0);
+      it.in = in;
+      it.slot = b.sorted[in];
+      out[t] = it;
+   }
+}
+
+__global__ void k_mat_fexchange(const BMember *__restrict__ mem, const int *__restrict__ pre,
+                                int M, int dir, int kv, BFxItem *__restrict__ out)
+{
+   const int total = pre[M];
+   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+      const int m = find_member(pre, M, t);
+      const BMember &b = mem[m];
+      const int local = t - pre[m], cnt = b.fcount[dir];
+      const int g = local / cnt, j = local - g * cnt;
+      BFxItem it;
+      it.fj = b.fjobs[b.fjoff[dir] + j];
+      it.m = m;
+      it.v0 = g * kv;
       out[t] = it;
    }
 }
@@ -161,6 +190,24 @@ __global__ __launch_bounds__(NT == 64 ? 256 : NT) void kb_exchange(
    }
 }
 
+// The same over the members' fused plans (a fused stage): `flip` says
+// which interior copy holds the cells (0: BMember::L, 1: BMember::Lo).
+template <typename T, int KV, int NT>
+__global__ __launch_bounds__(NT == 64 ? 256 : NT) void kb_exchange_fused(
+   const BMember *__restrict__ mem, const BFxItem *__restrict__ items, int total, int flip)
+{
+   const int tid = NT == 64 ? (int) (threadIdx.x & 63) : (int) threadIdx.x;
+   const int per = NT == 64 ? kWavesPerWg : 1;
+   const int first = NT == 64 ? first_item() : (int) blockIdx.x;
+   for (int t = first; t < total; t += gridDim.x * per) {
+      const BFxItem it = items[t];
+      const BMember &b = mem[it.m];
+      const Lay L = flip ? b.Lo : b.L;
+      exchange_job_fused<T, KV>((T *) b.pool, it.fj, b.edges, L, it.v0,
+                                min(KV, b.num_vars - it.v0), tid, NT);
+   }
+}
+
 // One stencil kernel per block-size class (the register needs of the
 // classes differ 2.5x, and a kernel holding every class's code would run
 // all of them at the largest one's occupancy).  Each walks the class's
@@ -177,10 +224,12 @@ template <> struct Cls<5> { enum { NC = 4, NP = 6, NZ = 0, D = 4 }; };
 
 // An item's layout is its member's (a scalar load: the member index is
 // uniform across the item's threads).
-template <typename T, int ST, int VPW, int C, bool WGM>
+// FUSED: the stencil of a fused stage, reading the copy `flip` names and
+// same-level faces in place, writing the other copy (as k_stencil_fused).
+template <typename T, int ST, int VPW, int C, bool WGM, bool FUSED = false>
 __global__ __launch_bounds__(256) void kb_stencil(const BMember *__restrict__ mem,
                                                   const BStItem *__restrict__ items, int total,
-                                                  int want_partials, int ring_elems)
+                                                  int want_partials, int ring_elems, int flip)
 {
    typedef typename Acc<T>::type A;
    typedef Cls<C> K;
@@ -190,9 +239,17 @@ __global__ __launch_bounds__(256) void kb_stencil(const BMember *__restrict__ me
       for (int t = blockIdx.x; t < total; t += gridDim.x) {
          const BStItem it = items[t];
          const BMember &b = mem[it.m];
-         stencil_block<T, ST, 1, 2, VPW, kThreads, K::NZ>((T *) it.rec, b.L, it.nv, (A *) smem,
-                                                          red, want_partials ? it.part : nullptr,
-                                                          b.part_stride);
+         if constexpr (FUSED) {
+            const Lay Lr = flip ? b.Lo : b.L, Lw = flip ? b.L : b.Lo;
+            stencil_block<T, ST, 1, 2, VPW, kThreads, K::NZ, kDepth, true>(
+               (T *) it.rec, Lr, it.nv, (A *) smem, red, want_partials ? it.part : nullptr,
+               b.part_stride, &Lw, it.slot, b.redir + 6 * (int64_t) it.in,
+               b.edges + (int64_t) kEdgeSrcs * it.slot);
+         } else {
+            stencil_block<T, ST, 1, 2, VPW, kThreads, K::NZ>((T *) it.rec, b.L, it.nv, (A *) smem,
+                                                             red, want_partials ? it.part : nullptr,
+                                                             b.part_stride);
+         }
       }
    } else {
       A *ring = (A *) smem + (threadIdx.x >> 6) * ring_elems;
@@ -200,9 +257,17 @@ __global__ __launch_bounds__(256) void kb_stencil(const BMember *__restrict__ me
       for (int t = first_item(); t < total; t += stride) {
          const BStItem it = items[t];
          const BMember &b = mem[it.m];
-         stencil_block<T, ST, K::NC, K::NP, VPW, 64, K::NZ, K::D>(
-            (T *) it.rec, b.L, it.nv, ring, nullptr, want_partials ? it.part : nullptr,
-            b.part_stride);
+         if constexpr (FUSED) {
+            const Lay Lr = flip ? b.Lo : b.L, Lw = flip ? b.L : b.Lo;
+            stencil_block<T, ST, K::NC, K::NP, VPW, 64, K::NZ, K::D, true>(
+               (T *) it.rec, Lr, it.nv, ring, nullptr, want_partials ? it.part : nullptr,
+               b.part_stride, &Lw, it.slot, b.redir + 6 * (int64_t) it.in,
+               b.edges + (int64_t) kEdgeSrcs * it.slot);
+         } else {
+            stencil_block<T, ST, K::NC, K::NP, VPW, 64, K::NZ, K::D>(
+               (T *) it.rec, b.L, it.nv, ring, nullptr, want_partials ? it.part : nullptr,
+               b.part_stride);
+         }
       }
    }
 }
@@ -244,6 +309,8 @@ struct amr_batch {
    int mem_cap = 0;
    int M = 0, plane_max = 0;
    int tot_ex[3] = {0, 0, 0}, tot_st[6] = {0, 0, 0, 0, 0, 0}, tot_rd = 0;
+   int tot_fx[3] = {0, 0, 0};
+   bool fusable = false;             // every member has a fused plan (two interiors)
    int plane_max_c[6] = {0, 0, 0, 0, 0, 0};
    bool table_ok = false;
    int table_vpw = 0, table_wg = -1, table_exv = 0;
@@ -258,13 +325,15 @@ struct amr_batch {
 
    // materialised items: exchange per direction, stencil per class
    BExItem *d_exitems = nullptr;
+   BFxItem *d_fxitems = nullptr;     // fused-stage exchange items per direction
+   size_t fx_cap = 0, fx_off[3] = {0, 0, 0};
    BStItem *d_stitems = nullptr;
    size_t ex_cap = 0, st_cap = 0;
    size_t ex_off[3] = {0, 0, 0}, st_off[6] = {0, 0, 0, 0, 0, 0};
 
    // graph
    hipGraphExec_t gexec = nullptr;
-   long gkey[10] = {0};
+   long gkey[11] = {0};
 
    // timing
    bool timing = false;
@@ -351,12 +420,14 @@ int ensure_table(amr_batch *b)
    const int M = (int) order.size();
    const int P = M + 1;
    std::vector<BMember> hm(M);
-   std::vector<int> pre(10 * P, 0);
+   std::vector<int> pre(13 * P, 0);
    int *ex[3] = {&pre[0], &pre[P], &pre[2 * P]};
    int *st[6];
    for (int c = 0; c < 6; c++)
       st[c] = &pre[(3 + c) * P];
    int *rd = &pre[9 * P];
+   int *fx[3] = {&pre[10 * P], &pre[11 * P], &pre[12 * P]};
+   bool fusable = M > 0;
    b->plane_max = 0;
    for (int c = 0; c < 6; c++)
       b->plane_max_c[c] = 0;
@@ -369,7 +440,16 @@ int ensure_table(amr_batch *b)
       m.sorted = d->d_sorted;
       m.partials = d->d_partials;
       m.sums = d->d_sums;
-      m.L = d->g;
+      if (int rc = amr_lay(d, 0, d->c.num_vars, &m.L))
+         return berr(b, rc, "member %d: %s", q, d->err);
+      m.Lo = amr_lay_other(d, m.L);
+      m.fjobs = d->d_fjobs;
+      m.redir = d->d_redir;
+      m.edges = d->d_edges;
+      fusable = fusable && d->two_interiors && !d->whole_ghosts && d->g.stencil != 0 &&
+                !(d->tape && d->tape->on) && (d->d_fjobs || !(d->fcount[0] + d->fcount[1] +
+                                                              d->fcount[2])) &&
+                d->d_redir && d->d_edges;
       m.n_active = d->n_active;
       m.num_vars = d->c.num_vars;
       m.part_stride = d->c.max_blocks;
@@ -380,6 +460,12 @@ int ensure_table(amr_batch *b)
          m.joff[dir] = off;
          off += d->count[dir];
          ex[dir][q + 1] = ex[dir][q] + d->count[dir] * ceil_div(m.num_vars, b->ex_vars);
+      }
+      for (int dir = 0, foff = 0; dir < 3; dir++) {
+         m.fcount[dir] = d->fcount[dir];
+         m.fjoff[dir] = foff;
+         foff += d->fcount[dir];
+         fx[dir][q + 1] = fx[dir][q] + d->fcount[dir] * ceil_div(m.num_vars, b->ex_vars);
       }
       const int items = d->n_active * ceil_div(m.num_vars, b->vpw);
       for (int c = 0; c < 6; c++)
@@ -399,11 +485,11 @@ int ensure_table(amr_batch *b)
       b->d_pre = nullptr;
       b->mem_cap = 0;
       BHIP(b, hipMalloc(&b->d_mem, sizeof(BMember) * cap));
-      BHIP(b, hipMalloc(&b->d_pre, sizeof(int) * 10 * (cap + 1)));
+      BHIP(b, hipMalloc(&b->d_pre, sizeof(int) * 13 * (cap + 1)));
       b->mem_cap = cap;
    }
    if (!b->d_pre)
-      BHIP(b, hipMalloc(&b->d_pre, sizeof(int) * 10));
+      BHIP(b, hipMalloc(&b->d_pre, sizeof(int) * 13));
    if (M)
       BHIP(b, hipMemcpy(b->d_mem, hm.data(), sizeof(BMember) * M, hipMemcpyHostToDevice));
    BHIP(b, hipMemcpy(b->d_pre, pre.data(), sizeof(int) * pre.size(), hipMemcpyHostToDevice));
@@ -413,6 +499,9 @@ int ensure_table(amr_batch *b)
    for (int c = 0; c < 6; c++)
       b->tot_st[c] = st[c][M];
    b->tot_rd = rd[M];
+   b->fusable = fusable;
+   for (int dir = 0; dir < 3; dir++)
+      b->tot_fx[dir] = fusable ? fx[dir][M] : 0;
    // expand the lists into item records
    size_t nex = 0, nst = 0;
    for (int dir = 0; dir < 3; dir++) {
@@ -430,6 +519,18 @@ int ensure_table(amr_batch *b)
       BHIP(b, hipMalloc(&b->d_exitems, sizeof(BExItem) * (nex + nex / 4)));
       b->ex_cap = nex + nex / 4;
    }
+   size_t nfx = 0;
+   for (int dir = 0; dir < 3; dir++) {
+      b->fx_off[dir] = nfx;
+      nfx += b->tot_fx[dir];
+   }
+   if (nfx > b->fx_cap) {
+      hipFree(b->d_fxitems);
+      b->d_fxitems = nullptr;
+      b->fx_cap = 0;
+      BHIP(b, hipMalloc(&b->d_fxitems, sizeof(BFxItem) * (nfx + nfx / 4)));
+      b->fx_cap = nfx + nfx / 4;
+   }
    if (nst > b->st_cap) {
       hipFree(b->d_stitems);
       b->d_stitems = nullptr;
@@ -442,6 +543,13 @@ int ensure_table(amr_batch *b)
          hipLaunchKernelGGL(k_mat_exchange, dim3(std::min(4096, ceil_div(b->tot_ex[dir], 256))),
                             dim3(256), 0, b->stream, b->d_mem, b->d_pre + dir * P, M, dir,
                             b->ex_vars, b->d_exitems + b->ex_off[dir]);
+         BHIP(b, hipGetLastError());
+      }
+   for (int dir = 0; dir < 3 && M; dir++)
+      if (b->tot_fx[dir]) {
+         hipLaunchKernelGGL(k_mat_fexchange, dim3(std::min(4096, ceil_div(b->tot_fx[dir], 256))),
+                            dim3(256), 0, b->stream, b->d_mem, b->d_pre + (10 + dir) * P, M, dir,
+                            b->ex_vars, b->d_fxitems + b->fx_off[dir]);
          BHIP(b, hipGetLastError());
       }
    for (int c = 0; c < 6 && M; c++)
@@ -469,8 +577,8 @@ int grid_of(int wgs, int total)
    return wgs > 0 ? std::min(wgs, need) : need;
 }
 
-template <typename T, int ST, int VPW, int C>
-int launch_class(amr_batch *b, int want_partials)
+template <typename T, int ST, int VPW, int C, bool FUSED>
+int launch_class(amr_batch *b, int want_partials, int flip)
 {
    typedef typename Acc<T>::type A;
    const int total = b->tot_st[C];
@@ -480,37 +588,86 @@ int launch_class(amr_batch *b, int want_partials)
    if ((b->wg_mask >> (C / 2)) & 1) {
       const size_t lds = sizeof(A) * 3 * VPW * (size_t) b->plane_max_c[C];
       const int grid = b->st_wgs > 0 ? std::min(b->st_wgs, total) : total;
-      hipLaunchKernelGGL((kb_stencil<T, ST, VPW, C, true>), dim3(grid), dim3(256), lds,
-                         b->stream, b->d_mem, items, total, want_partials, 0);
+      hipLaunchKernelGGL((kb_stencil<T, ST, VPW, C, true, FUSED>), dim3(grid), dim3(256), lds,
+                         b->stream, b->d_mem, items, total, want_partials, 0, flip);
    } else {
       const int ring = 3 * VPW * b->plane_max_c[C];
       const size_t lds = sizeof(A) * (size_t) ring * kWavesPerWg;
-      hipLaunchKernelGGL((kb_stencil<T, ST, VPW, C, false>), dim3(grid_of(b->st_wgs, total)),
-                         dim3(256), lds, b->stream, b->d_mem, items, total, want_partials, ring);
+      hipLaunchKernelGGL((kb_stencil<T, ST, VPW, C, false, FUSED>),
+                         dim3(grid_of(b->st_wgs, total)), dim3(256), lds, b->stream, b->d_mem,
+                         items, total, want_partials, ring, flip);
    }
    BHIP(b, hipGetLastError());
    return AMR_OK;
 }
 
-template <typename T, int ST, int VPW>
-int launch_stencil_t(amr_batch *b, int want_partials)
+template <typename T, int ST, int VPW, bool FUSED>
+int launch_stencil_t(amr_batch *b, int want_partials, int flip)
 {
-   int rc = launch_class<T, ST, VPW, 4>(b, want_partials);
-   if (!rc) rc = launch_class<T, ST, VPW, 5>(b, want_partials);
-   if (!rc) rc = launch_class<T, ST, VPW, 2>(b, want_partials);
-   if (!rc) rc = launch_class<T, ST, VPW, 3>(b, want_partials);
-   if (!rc) rc = launch_class<T, ST, VPW, 0>(b, want_partials);
-   if (!rc) rc = launch_class<T, ST, VPW, 1>(b, want_partials);
+   int rc = launch_class<T, ST, VPW, 4, FUSED>(b, want_partials, flip);
+   if (!rc) rc = launch_class<T, ST, VPW, 5, FUSED>(b, want_partials, flip);
+   if (!rc) rc = launch_class<T, ST, VPW, 2, FUSED>(b, want_partials, flip);
+   if (!rc) rc = launch_class<T, ST, VPW, 3, FUSED>(b, want_partials, flip);
+   if (!rc) rc = launch_class<T, ST, VPW, 0, FUSED>(b, want_partials, flip);
+   if (!rc) rc = launch_class<T, ST, VPW, 1, FUSED>(b, want_partials, flip);
    return rc;
 }
 
+template <typename T, bool FUSED>
+int launch_stencil_st(amr_batch *b, int want_partials, int flip)
+{
+   if (b->stencil == 7)
+      return b->vpw == 2 ? launch_stencil_t<T, 7, 2, FUSED>(b, want_partials, flip)
+                         : launch_stencil_t<T, 7, 1, FUSED>(b, want_partials, flip);
+   return b->vpw == 2 ? launch_stencil_t<T, 27, 2, FUSED>(b, want_partials, flip)
+                      : launch_stencil_t<T, 27, 1, FUSED>(b, want_partials, flip);
+}
+
+// The exchange of a fused stage: every member's fused plan, the cells in
+// the copy `flip` names.
 template <typename T>
-int enqueue_stage_t(amr_batch *b, int stage, int permute, int row)
+int enqueue_fused_exchange(amr_batch *b, int flip)
+{
+   for (int dir = 0; dir < 3; dir++) {
+      const int total = b->tot_fx[dir];
+      if (!total)
+         continue;
+      const BFxItem *items = b->d_fxitems + b->fx_off[dir];
+      if (b->ex_threads == 64) {
+         auto k = b->ex_vars == 16 ? kb_exchange_fused<T, 16, 64>
+                : b->ex_vars == 4 ? kb_exchange_fused<T, 4, 64> : kb_exchange_fused<T, 8, 64>;
+         hipLaunchKernelGGL(k, dim3(grid_of(b->ex_wgs, total)), dim3(256), 0, b->stream,
+                            b->d_mem, items, total, flip);
+      } else {
+         const int grid = b->ex_wgs > 0 ? std::min(b->ex_wgs, total) : total;
+         auto k = b->ex_threads == 256
+                     ? (b->ex_vars == 16 ? kb_exchange_fused<T, 16, 256>
+                        : b->ex_vars == 4 ? kb_exchange_fused<T, 4, 256>
+                                          : kb_exchange_fused<T, 8, 256>)
+                     : (b->ex_vars == 16 ? kb_exchange_fused<T, 16, 128>
+                        : b->ex_vars == 4 ? kb_exchange_fused<T, 4, 128>
+                                          : kb_exchange_fused<T, 8, 128>);
+         hipLaunchKernelGGL(k, dim3(grid), dim3(b->ex_threads), 0, b->stream, b->d_mem, items,
+                            total, flip);
+      }
+      BHIP(b, hipGetLastError());
+   }
+   return AMR_OK;
+}
+
+// One stage; fused (every member's fused plan, reading the copy `flip`
+// names and writing the other) or plain.
+template <typename T>
+int enqueue_stage_t(amr_batch *b, int stage, int permute, int row, bool fused, int flip)
 {
    static const int perm[6][3] = { {0, 1, 2}, {1, 2, 0}, {2, 0, 1},
                                    {0, 2, 1}, {1, 0, 2}, {2, 1, 0} };
    span_begin(b, 0);
-   for (int o = 0; o < 3; o++) {
+   if (fused) {
+      if (int rc = enqueue_fused_exchange<T>(b, flip))
+         return rc;
+   }
+   for (int o = 0; o < 3 && !fused; o++) {
       const int dir = permute ? perm[stage % 6][o] : o;
       if (!b->tot_ex[dir])
          continue;
@@ -539,11 +696,8 @@ int enqueue_stage_t(amr_batch *b, int stage, int permute, int row)
    if (tot_st) {
       span_begin(b, 1);
       const int wp = row >= 0;
-      int rc;
-      if (b->stencil == 7)
-         rc = b->vpw == 2 ? launch_stencil_t<T, 7, 2>(b, wp) : launch_stencil_t<T, 7, 1>(b, wp);
-      else
-         rc = b->vpw == 2 ? launch_stencil_t<T, 27, 2>(b, wp) : launch_stencil_t<T, 27, 1>(b, wp);
+      const int rc = fused ? launch_stencil_st<T, true>(b, wp, flip)
+                           : launch_stencil_st<T, false>(b, wp, 0);
       span_end(b);
       if (rc)
          return rc;
@@ -558,13 +712,43 @@ int enqueue_stage_t(amr_batch *b, int stage, int permute, int row)
    return AMR_OK;
 }
 
-int enqueue_stage(amr_batch *b, int stage, int permute, int row)
+int enqueue_stage(amr_batch *b, int stage, int permute, int row, bool fused = false,
+                  int flip = 0)
 {
    switch (b->precision) {
-   case AMR_PREC_F64: return enqueue_stage_t<double>(b, stage, permute, row);
-   case AMR_PREC_F32: return enqueue_stage_t<float>(b, stage, permute, row);
-   default: return enqueue_stage_t<uint16_t>(b, stage, permute, row);
+   case AMR_PREC_F64: return enqueue_stage_t<double>(b, stage, permute, row, fused, flip);
+   case AMR_PREC_F32: return enqueue_stage_t<float>(b, stage, permute, row, fused, flip);
+   default: return enqueue_stage_t<uint16_t>(b, stage, permute, row, fused, flip);
    }
+}
+
+// MINIAMR_BATCH_FUSED=0 keeps the batch's stages plain.
+bool batch_fused_on()
+{
+   static int v = -1;
+   if (v < 0) {
+      const char *e = getenv("MINIAMR_BATCH_FUSED");
+      v = e ? atoi(e) != 0 : 1;
+   }
+   return v != 0;
+}
+
+// A timestep of `nstages` stages runs fused when every member has a fused
+// plan and the count is even (so each interior ends in the copy it started
+// in, which the work lists and the captured graph assume).
+bool fuse_timestep(const amr_batch *b, int nstages, int permute)
+{
+   return b->fusable && !permute && nstages % 2 == 0 && batch_fused_on();
+}
+
+// After fused stages: the members' ghost planes of in-place faces are
+// stale (amr_require_fresh brings them up to date when a member call needs
+// them).
+void mark_stale(amr_batch *b, bool stale)
+{
+   for (size_t i = 0; i < b->members.size(); i++)
+      if (b->active[i])
+         std::fill(b->members[i]->stale.begin(), b->members[i]->stale.end(), stale ? 1 : 0);
 }
 
 int check_row(amr_batch *b, int row)
@@ -596,10 +780,14 @@ int time_stages(amr_batch *b, int reps, double *ms)
    hipEvent_t e0 = bevent(b), e1 = bevent(b);
    if (!e0 || !e1)
       return berr(b, AMR_ERR_HIP, "autotune: no events");
-   rc = enqueue_stage(b, 0, 0, -1);
+   // the stages a timestep would run: fused ones when the timesteps are
+   // (warm-up plus reps stages, an even count, so each interior ends in
+   // the copy it started in)
+   const bool fused = fuse_timestep(b, reps + 1, 0);
+   rc = enqueue_stage(b, 0, 0, -1, fused, 0);
    hipEventRecord(e0, b->stream);
    for (int r = 0; r < reps && !rc; r++)
-      rc = enqueue_stage(b, r, 0, -1);
+      rc = enqueue_stage(b, r, 0, -1, fused, (r + 1) & 1);
    hipEventRecord(e1, b->stream);
    b->timing = tm;
    BHIP(b, hipEventSynchronize(e1));
@@ -664,6 +852,7 @@ int amr_batch_destroy(amr_batch *b)
    hipFree(b->d_mem);
    hipFree(b->d_pre);
    hipFree(b->d_exitems);
+   hipFree(b->d_fxitems);
    hipFree(b->d_stitems);
    if (b->stream)
       hipStreamDestroy(b->stream);
@@ -766,6 +955,8 @@ int amr_batch_stage(amr_batch *b, int stage, int permute, int sum_row)
       rc = check_row(b, sum_row);
    if (!rc)
       rc = enqueue_stage(b, stage, permute, sum_row);
+   if (!rc)
+      mark_stale(b, false);   // a plain stage writes every ghost plane
    return rc;
 }
 
@@ -780,9 +971,10 @@ int amr_batch_timestep(amr_batch *b, int stage0, int nstages, int checksum_freq,
       return rc;
    if (checksum_freq && (rc = check_row(b, (nstages - 1) / checksum_freq)))
       return rc;
-   const long key[10] = {b->epoch, permute ? stage0 % 6 : -1, nstages, checksum_freq,
+   const bool fused = fuse_timestep(b, nstages, permute);
+   const long key[11] = {b->epoch, permute ? stage0 % 6 : -1, nstages, checksum_freq,
                          b->vpw, b->st_wgs, b->ex_wgs, (long) b->wg_mask, (long) b->ex_vars,
-                         (long) b->ex_threads};
+                         (long) b->ex_threads, (long) fused};
    if (!b->gexec || memcmp(key, b->gkey, sizeof(key))) {
       drop_graph(b);
       const bool tm = b->timing;
@@ -790,7 +982,7 @@ int amr_batch_timestep(amr_batch *b, int stage0, int nstages, int checksum_freq,
       BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
       for (int s = 0; s < nstages && !rc; s++) {
          const int row = (checksum_freq && !(s % checksum_freq)) ? s / checksum_freq : -1;
-         rc = enqueue_stage(b, stage0 + s, permute, row);
+         rc = enqueue_stage(b, stage0 + s, permute, row, fused, s & 1);
       }
       hipGraph_t graph = nullptr;
       hipError_t e = hipStreamEndCapture(b->stream, &graph);
@@ -812,6 +1004,8 @@ int amr_batch_timestep(amr_batch *b, int stage0, int nstages, int checksum_freq,
    span_begin(b, 3);
    BHIP(b, hipGraphLaunch(b->gexec, b->stream));
    span_end(b);
+   // fused: ghost planes of in-place faces stale; plain: all written
+   mark_stale(b, fused);
    return AMR_OK;
 }
 
@@ -845,7 +1039,11 @@ int amr_batch_autotune(amr_batch *b, double *ms_out)
    }
    // coordinate descent: workgroup/wavefront items per size class, then
    // variables per stencil item, then the persistent grids
-   const int reps = 3;
+   const int reps = 3;   // + 1 warm-up: an even number of stages per trial
+   // the members' stale flags, restored with the pools
+   std::vector<std::vector<uint8_t>> stale0;
+   for (size_t i = 0; i < b->members.size(); i++)
+      stale0.push_back(b->members[i]->stale);
    const int mult[5] = {0, 1, 2, 4, 8};
    double best = 1e30;
    auto trial = [&](int vpw, int st, int ex, int wgm, int exv, int ext, bool *better) {
@@ -895,6 +1093,8 @@ int amr_batch_autotune(amr_batch *b, double *ms_out)
    }
    hipError_t e = hipStreamSynchronize(b->stream);
    hipFree(backup);
+   for (size_t i = 0; i < b->members.size(); i++)
+      b->members[i]->stale = stale0[i];
    drop_graph(b);
    if (rc)
       return rc;

@@ -100,3 +100,19 @@ def test_batch_shapes_do_not_change_results(exes, shape):
     assert a["rc"] == 0 and c["rc"] == 0
     for i in range(len(runs)):
         A.same_run(a["runs"][i], c["runs"][i], sum_rtol=1e-13, max_ulp=0)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_batch_fused_and_plain_stages_agree(exes, prec):
+    """27-point timesteps of an even number of stages run fused (same-level
+    faces read in place, as amr_dev_stage_fused) unless MINIAMR_BATCH_FUSED=0;
+    both give the same cells, ghosts included (the stale ghost planes of
+    in-place faces are brought up to date before each refinement and before
+    the dumps), and checksums up to the association of the partial sums."""
+    print("case: batch %s fused vs plain" % prec, flush=True)
+    runs = [r + COMMON27 + " --precision " + prec for r in RUNS]
+    a = A.run_batch(exes, runs, dump="cells", timeout=300)
+    c = A.run_batch(exes, runs, dump="cells", timeout=300, env={"MINIAMR_BATCH_FUSED": "0"})
+    assert a["rc"] == 0 and c["rc"] == 0
+    for i in range(len(runs)):
+        A.same_run(a["runs"][i], c["runs"][i], sum_rtol=1e-13, max_ulp=0)
