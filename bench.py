@@ -494,7 +494,7 @@ def main():
     global TIMING_LEVEL
     TIMING_LEVEL = a.timing_level
     if a.gpus != 1:
-        print("bench.py: this build runs one rank on one GPU; --gpus %d is not supported "
+        print("bench.py: the benchmark runs one rank on one GPU; --gpus %d is not supported "
               "(blocks would shard across GPUs as the reference's ranks do; not built)" % a.gpus,
               file=sys.stderr)
         sys.exit(2)

@@ -42,7 +42,7 @@ typedef struct {
    int refine_ghost, change_dir, group_blocks, limit_move, send_faces;
    int lb_method;
    amr_object *objects;
-   /* Extensions of this build (environment, not command line). */
+   /* Extensions beyond the reference (environment, not command line). */
    int precision;        /* AMR_PREC_* (MINIAMR_PRECISION=f64|f32|bf16) */
    int schedule;         /* 0: stage-major like the reference,
                             1: variable-major within a refinement interval */

@@ -74,7 +74,7 @@ void amr_print_help(void)
       "--num_objects - (>= 0) number of objects to cause refinement",
       "--object - type, bounce, position(3), movement(3), size(3), size rate of change(3)",
       "",
-      "Extensions of this build (not in the reference):",
+      "Extensions (not in the reference):",
       "--seed - seed of the initial values' generator (default 1, the reference's)",
       "--precision - storage of the cell values: f64 (default), f32 or bf16",
       "--tape - record the device's data operations for the reverse pass (amr_adjoint.h)",
@@ -223,7 +223,7 @@ int amr_parse_args(amr_params *P, int argc, char **argv)
    return 0;
 }
 
-/* Validation of ref/main.c:657-723 for one rank, plus this build's scope. */
+/* Validation of ref/main.c:657-723 for one rank, plus what this driver leaves out. */
 int amr_check_input(amr_params *P)
 {
    int error = 0;
@@ -274,12 +274,12 @@ int amr_check_input(amr_params *P)
    if (P->lb_opt < 0 || P->lb_opt > 2) {
       error = amr_note_error("lb_opt must be 0, 1, or 2\n");
    }
-   /* scope of this build */
+   /* what this driver leaves out */
    if (!error && P->lb_method >= 2) {
-      error = amr_note_error("this build implements the RCB/Morton initial block order only (not --hilbert)\n");
+      error = amr_note_error("miniAMR.x implements the RCB/Morton initial block order only (not --hilbert)\n");
    }
    if (!error && P->plot_freq) {
-      error = amr_note_error("this build does not write plot files (--plot_freq must be 0)\n");
+      error = amr_note_error("miniAMR.x does not write plot files (--plot_freq must be 0)\n");
    }
    return error;
 }

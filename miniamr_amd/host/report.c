@@ -17,7 +17,7 @@
  *             comm call (same level 2 per pair, coarse/fine 2 per quarter
  *             pair, 1 per boundary face), refinement steps, blocks split
  *             and reformed, floating-point operations of the stencils;
- *   zeros     what only exists between ranks or in phases this build does
+ *   zeros     what only exists between ranks or in phases this driver does
  *             not have: messages, bytes, packing, waits, partners, load
  *             balance, redistribution, plots;
  *   memory    the device context's buffers (amr_dev_memory) where the

@@ -1,6 +1,6 @@
 """End-of-run report comparison (results.yaml, results.txt, stdout).
 
-The reports of the reference (ref/profile.c:37-1300) and of this build
+The reports of the reference (ref/profile.c:37-1300) and of the drop-in
 print the same lines; the numbers that measure time or memory differ by
 nature (host wall time against GPU device time, mallocs against device
 buffers), every other number -- configuration, object motion, faces
