@@ -100,41 +100,23 @@ int amr_dev_create(const amr_dev_config *cfg, amr_dev **out)
    if (int rc = amr_noalloc_init(d))   // static scratch of amr_dev_split_noalloc
       return rc;
 
-   Lay &g = d->g;
-   g.nx = cfg->nx; g.ny = cfg->ny; g.nz = cfg->nz;
-   g.gx = g.nx + 2; g.gy = g.ny + 2; g.gz = g.nz + 2;
-   g.stencil = cfg->stencil;
-   int o = g.nx * g.ny * g.nz;                       // interior first
-   g.oF[0] = o; o += g.ny * g.nz;
-   g.oF[1] = o; o += g.ny * g.nz;
-   g.oF[2] = o; o += g.nx * g.nz;
-   g.oF[3] = o; o += g.nx * g.nz;
-   g.oF[4] = o; o += g.nx * g.ny;
-   g.oF[5] = o; o += g.nx * g.ny;
-   g.oEXY = o; o += 4 * g.gz;
-   g.oEXZ = o; o += 4 * g.ny;
-   g.oEYZ = o; o += 4 * g.nx;
-   g.oZI = o; o += 2 * g.nx * g.ny;
-   g.oZX = o; o += 4 * g.gy;
-   g.oZY = o; o += 4 * g.nx;
-   g.oI = 0;
-   d->oI_par[0] = 0;
-   d->oZI_par[0] = d->oZI_par[1] = g.oZI;
+   d->elem_bytes = cfg->precision == AMR_PREC_F64 ? 8 : cfg->precision == AMR_PREC_F32 ? 4 : 2;
    const char *fz = getenv("MINIAMR_FUSED");
    d->fused_mode = fz ? atoi(fz) : 1;
    if (cfg->stencil == 0)
       d->fused_mode = 0;   // the variable-work stencils read other variables' faces
-   d->two_interiors = d->fused_mode != 0;
-   if (d->two_interiors) {                           // second I and ZI
-      d->oI_par[1] = o; o += g.nx * g.ny * g.nz;
-      d->oZI_par[1] = o; o += 2 * g.nx * g.ny;
+   d->two_interiors = d->fused_mode != 0;   // second I and ZI
+   const RecordLayout rl = make_record_layout(cfg->nx, cfg->ny, cfg->nz, cfg->stencil,
+                                              d->elem_bytes, d->two_interiors);
+   Lay &g = d->g;
+   g = rl.g;
+   for (int p = 0; p < 2; p++) {
+      d->oI_par[p] = rl.oI_par[p];
+      d->oZI_par[p] = rl.oZI_par[p];
    }
    d->par.assign(cfg->num_vars, 0);
    d->stale.assign(cfg->num_vars, 0);
-   d->elem_bytes = cfg->precision == AMR_PREC_F64 ? 8 : cfg->precision == AMR_PREC_F32 ? 4 : 2;
-   int64_t per256 = 256 / d->elem_bytes;
    int64_t cells = (int64_t) g.gx * g.gy * g.gz;
-   g.rec = ((int64_t) o + per256 - 1) / per256 * per256;
    g.var_stride = g.rec * cfg->max_blocks;
    size_t pool_bytes = (size_t) g.var_stride * cfg->num_vars * d->elem_bytes;
 
