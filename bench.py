@@ -595,9 +595,9 @@ def main():
                      "traffic_frac": (round(traffic / (stencil_s / launches) / 1e9 /
                                             HBM_PEAK_GBS, 4)
                                       if traffic and launches and stencil_s > 0 else None),
-                     "kernel": "k_stencil_v16 (27-point, 16x16 faces; every stage but "
-                               "a timestep's last reads same-level faces in place), %d "
-                               "launches, %.3f ms avg" % (
+                     "kernel": "k_stencil_v16 (27-point, 16x16 faces; every stage reads "
+                               "same-level faces in place but the last before a refinement "
+                               "or the run's end), %d launches, %.3f ms avg" % (
                                    launches, stencil_s / max(1, launches) * 1e3),
                      # the practical ceiling on this GPU: a streaming copy
                      # and read (amr_dev_copy_bandwidth, 4 GiB, best of 3),

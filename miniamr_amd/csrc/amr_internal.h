@@ -295,9 +295,9 @@ template <typename T> __device__ __forceinline__ void st_stream(gptr<T> base, ui
 // with a second I and ZI when two_interiors (oI_par[1] / oZI_par[1]; else
 // both entries name the first copy).  Every part starts on a 128-byte line,
 // so a 16-cell row of the interior or of a z-plane copy never straddles two
-// lines (with ZI 64 bytes into a line, the fused FP64 stencil ran ~4% slower
-// in the stages that wrote that copy); the parts the stencil streams (I,
-// F, ZI) come first.  rec is padded to 256 bytes; var_stride is left to the
+// lines (with the first ZI 64 bytes into a line, the fused FP64 stencil ran
+// ~4% slower in every other stage, as the stages take the two interior
+// copies in turn); the parts the stencil streams (I, F, ZI) come first.  rec is padded to 256 bytes; var_stride is left to the
 // caller.
 struct RecordLayout {
    Lay g;
