@@ -186,6 +186,9 @@ struct amr_dev {
    // remaining jobs read their cells where the extension would copy from)
    EdgeSrc *d_edges = nullptr;
    bool fold_edges = false;
+   // pinned staging of set_mesh's plan uploads (stream-ordered copies)
+   void *h_plan = nullptr;
+   size_t h_plan_bytes = 0;
    std::vector<uint8_t> stale;
 
    Tape *tape = nullptr;            // recording of data operations, if any

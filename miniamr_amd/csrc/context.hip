@@ -173,6 +173,8 @@ int amr_dev_destroy(amr_dev *d)
    hipFree(d->mk_out);
    if (d->h_pinned)
       hipHostFree(d->h_pinned);
+   if (d->h_plan)
+      hipHostFree(d->h_plan);
    if (d->stream && d->owns_stream)
       hipStreamDestroy(d->stream);
    amr_tape_free(d);
@@ -407,8 +409,19 @@ struct EdgeResolver {
 // y/z for the 7-point stencil) or reduced to its edge extension (y/z for
 // the 27-point stencil); every job that reads a source's ghost row that is
 // read in place is redirected to the block that row would be copied from.
-static int build_fused_plan(amr_dev *d, const int32_t *sorted, int n_active,
-                            const amr_face_job *jobs, const int32_t count[3])
+// It is computed on the host only; set_mesh uploads it.
+struct FusedPlan {
+   std::vector<FusedJob> fj;
+   int32_t fc[3] = {0, 0, 0};
+   std::vector<EdgeSrc> edges;
+   std::vector<int32_t> redir;
+   std::vector<amr_face_job> mj;
+   int32_t mc[3] = {0, 0, 0};
+   bool fold = false;
+};
+
+static void compute_fused_plan(const amr_dev *d, const int32_t *sorted, int n_active,
+                               const amr_face_job *jobs, const int32_t count[3], FusedPlan &P)
 {
    const int M = d->c.max_blocks;
    std::vector<int32_t> same(6 * (size_t) M, -1);
@@ -429,7 +442,8 @@ static int build_fused_plan(amr_dev *d, const int32_t *sorted, int n_active,
          inp[6 * (size_t) sorted[in] + f] = inplace(sorted[in], f);
    // edge sources of every active block: folded (27-point stencil) unless a
    // line is not one affine run, in which case the plan keeps the extensions
-   std::vector<EdgeSrc> edges(kEdgeSrcs * (size_t) M, EdgeSrc{0, 0, 0, ES_ABS});
+   std::vector<EdgeSrc> &edges = P.edges;
+   edges.assign(kEdgeSrcs * (size_t) M, EdgeSrc{0, 0, 0, ES_ABS});
    bool fold = d->g.stencil == 27 && fold_edges_on();
    {
       // the blocks are independent: spread them over a few host threads
@@ -461,7 +475,7 @@ static int build_fused_plan(amr_dev *d, const int32_t *sorted, int n_active,
       if (!fold)
          run_all(false);
    }
-   d->fold_edges = fold;
+   P.fold = fold;
    // Jobs by launch.  The y and z prolongations and restrictions of blocks
    // without a boundary face go with the x phase: they read only interior
    // cells, which no job writes, and write ghost cells that only later
@@ -497,59 +511,24 @@ static int build_fused_plan(amr_dev *d, const int32_t *sorted, int n_active,
          ph[hoist && dir > 0 && cf && !bc[j.dst] ? 0 : dir].push_back(f);
       }
    // the jobs that bring stale ghost planes up to date (amr_require_fresh)
-   std::vector<amr_face_job> mj;
-   int32_t mc[3] = {0, 0, 0};
+   std::vector<amr_face_job> &mj = P.mj;
+   int32_t *mc = P.mc;
    for (int t = 0, dir = 0; dir < 3; dir++)
       for (int e = 0; e < count[dir]; e++, t++)
          if (jobs[t].kind == AMR_FACE_SAME && inplace(jobs[t].dst, jobs[t].face) >= 0) {
             mj.push_back(jobs[t]);
             mc[dir]++;
          }
-   std::vector<FusedJob> fj;
+   std::vector<FusedJob> &fj = P.fj;
    fj.reserve(nj);
-   int32_t fc[3];
    for (int p = 0; p < 3; p++) {
       fj.insert(fj.end(), ph[p].begin(), ph[p].end());
-      fc[p] = (int32_t) ph[p].size();
+      P.fc[p] = (int32_t) ph[p].size();
    }
-   std::vector<int32_t> redir(6 * (size_t) std::max(n_active, 1), -1);
+   P.redir.assign(6 * (size_t) std::max(n_active, 1), -1);
    for (int in = 0; in < n_active; in++)
       for (int f = 0; f < 6; f++)
-         redir[6 * (size_t) in + f] = inplace(sorted[in], f);
-   if ((int) fj.size() > d->fjobs_cap) {
-      hipFree(d->d_fjobs);
-      d->d_fjobs = nullptr;
-      int cap = std::max((int) fj.size(), 2 * d->fjobs_cap);
-      HIP_TRY(d, hipMalloc(&d->d_fjobs, sizeof(FusedJob) * cap));
-      d->fjobs_cap = cap;
-   }
-   if (!fj.empty())
-      HIP_TRY(d, hipMemcpy(d->d_fjobs, fj.data(), sizeof(FusedJob) * fj.size(),
-                           hipMemcpyHostToDevice));
-   if (!d->d_edges)
-      HIP_TRY(d, hipMalloc(&d->d_edges, sizeof(EdgeSrc) * edges.size()));
-   HIP_TRY(d, hipMemcpy(d->d_edges, edges.data(), sizeof(EdgeSrc) * edges.size(),
-                        hipMemcpyHostToDevice));
-   if (!d->d_redir)
-      HIP_TRY(d, hipMalloc(&d->d_redir, sizeof(int32_t) * 6 * (size_t) M));
-   HIP_TRY(d, hipMemcpy(d->d_redir, redir.data(), sizeof(int32_t) * redir.size(),
-                        hipMemcpyHostToDevice));
-   for (int i = 0; i < 3; i++)
-      d->fcount[i] = fc[i];
-   if ((int) mj.size() > d->mjobs_cap) {
-      const int cap = std::max((int) mj.size(), 2 * d->mjobs_cap);
-      hipFree(d->d_mjobs);
-      d->d_mjobs = nullptr;
-      d->mjobs_cap = 0;
-      HIP_TRY(d, hipMalloc(&d->d_mjobs, sizeof(amr_face_job) * cap));
-      d->mjobs_cap = cap;
-   }
-   if (!mj.empty())
-      HIP_TRY(d, hipMemcpy(d->d_mjobs, mj.data(), sizeof(amr_face_job) * mj.size(),
-                           hipMemcpyHostToDevice));
-   for (int i = 0; i < 3; i++)
-      d->mcount[i] = mc[i];
-   return AMR_OK;
+         P.redir[6 * (size_t) in + f] = inplace(sorted[in], f);
 }
 
 // A fused stage leaves unwritten the ghost cells its stencil reads where
@@ -616,20 +595,75 @@ int amr_set_mesh_impl(amr_dev *d, const int32_t *sorted, int n_active,
           j.src < 0 || j.src >= d->c.max_blocks || (j.iq & ~1) || (j.jq & ~1))
          return amr_set_error(d, AMR_ERR_ARG, "set_mesh: bad face job %d", t);
    }
+   // The fused plan is host work; it runs before the wait below, so it
+   // overlaps the device work still queued (split and consolidate data).
+   FusedPlan fp;
+   if (d->two_interiors)
+      compute_fused_plan(d, sorted, n_active, jobs, count, fp);
+   // Queued work may still read the old plan and the staging buffer may
+   // still feed the last set_mesh's copies: wait, then replace.
    HIP_TRY(d, hipStreamSynchronize(d->stream));
-   if (n_active)
-      HIP_TRY(d, hipMemcpy(d->d_sorted, sorted, sizeof(int32_t) * n_active,
-                           hipMemcpyHostToDevice));
-   d->h_sorted.assign(sorted, sorted + n_active);
-   if (nj > d->jobs_cap) {
-      hipFree(d->d_jobs);
-      d->d_jobs = nullptr;
-      int cap = std::max(nj, 2 * d->jobs_cap);
-      HIP_TRY(d, hipMalloc(&d->d_jobs, sizeof(amr_face_job) * cap));
-      d->jobs_cap = cap;
+   auto grow = [&](void **buf, int *cap, int need, size_t elem) -> int {
+      if (need <= *cap)
+         return AMR_OK;
+      const int c = std::max(need, 2 * *cap);
+      hipFree(*buf);
+      *buf = nullptr;
+      *cap = 0;
+      HIP_TRY(d, hipMalloc(buf, elem * (size_t) c));
+      *cap = c;
+      return AMR_OK;
+   };
+   if (int rc = grow((void **) &d->d_jobs, &d->jobs_cap, nj, sizeof(amr_face_job)))
+      return rc;
+   if (d->two_interiors) {
+      if (int rc = grow((void **) &d->d_fjobs, &d->fjobs_cap, (int) fp.fj.size(), sizeof(FusedJob)))
+         return rc;
+      if (int rc = grow((void **) &d->d_mjobs, &d->mjobs_cap, (int) fp.mj.size(),
+                        sizeof(amr_face_job)))
+         return rc;
+      if (!d->d_edges)
+         HIP_TRY(d, hipMalloc(&d->d_edges, sizeof(EdgeSrc) * fp.edges.size()));
+      if (!d->d_redir)
+         HIP_TRY(d, hipMalloc(&d->d_redir, sizeof(int32_t) * 6 * (size_t) d->c.max_blocks));
    }
-   if (nj)
-      HIP_TRY(d, hipMemcpy(d->d_jobs, jobs, sizeof(amr_face_job) * nj, hipMemcpyHostToDevice));
+   // One pinned staging buffer, then copies ordered on the stream before
+   // any later launch (pageable hipMemcpy of the ~3 MB plan took 0.5-2 ms).
+   struct Piece {
+      void *dst;
+      const void *src;
+      size_t bytes;
+   };
+   std::vector<Piece> pieces;
+   pieces.push_back({d->d_sorted, sorted, sizeof(int32_t) * (size_t) n_active});
+   pieces.push_back({d->d_jobs, jobs, sizeof(amr_face_job) * (size_t) nj});
+   if (d->two_interiors) {
+      pieces.push_back({d->d_fjobs, fp.fj.data(), sizeof(FusedJob) * fp.fj.size()});
+      pieces.push_back({d->d_edges, fp.edges.data(), sizeof(EdgeSrc) * fp.edges.size()});
+      pieces.push_back({d->d_redir, fp.redir.data(), sizeof(int32_t) * fp.redir.size()});
+      pieces.push_back({d->d_mjobs, fp.mj.data(), sizeof(amr_face_job) * fp.mj.size()});
+   }
+   size_t total = 0;
+   for (const Piece &q : pieces)
+      total += (q.bytes + 255) / 256 * 256;
+   if (total > d->h_plan_bytes) {
+      if (d->h_plan)
+         hipHostFree(d->h_plan);
+      d->h_plan = nullptr;
+      d->h_plan_bytes = 0;
+      HIP_TRY(d, hipHostMalloc(&d->h_plan, total, hipHostMallocDefault));
+      d->h_plan_bytes = total;
+   }
+   size_t at = 0;
+   for (const Piece &q : pieces) {
+      if (!q.bytes)
+         continue;
+      char *h = (char *) d->h_plan + at;
+      memcpy(h, q.src, q.bytes);
+      HIP_TRY(d, hipMemcpyAsync(q.dst, h, q.bytes, hipMemcpyHostToDevice, d->stream));
+      at += (q.bytes + 255) / 256 * 256;
+   }
+   d->h_sorted.assign(sorted, sorted + n_active);
    d->n_active = n_active;
    for (int i = 0; i < 3; i++)
       d->count[i] = count[i];
@@ -637,8 +671,13 @@ int amr_set_mesh_impl(amr_dev *d, const int32_t *sorted, int n_active,
    for (int t = 0; t < nj; t++)
       d->whole_ghosts |= (jobs[t].kind & AMR_FACE_GHOSTS) != 0;
    d->plan_version++;
-   if (d->two_interiors)
-      return build_fused_plan(d, sorted, n_active, jobs, count);
+   if (d->two_interiors) {
+      d->fold_edges = fp.fold;
+      for (int i = 0; i < 3; i++) {
+         d->fcount[i] = fp.fc[i];
+         d->mcount[i] = fp.mc[i];
+      }
+   }
    return AMR_OK;
 }
 

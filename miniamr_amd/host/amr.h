@@ -103,6 +103,8 @@ typedef struct amr_run {
    int64_t *sorted_number;     /* sorted list (ref/block.h:72-77) */
    int *sorted_slot;
    int *sorted_index;          /* num_refine + 2 entries */
+   amr_vec sorted_new;         /* slots given a block since the last rebuild */
+   unsigned char *sorted_mark; /* max_num_blocks flags, zero between rebuilds */
    int64_t *num_blocks;        /* blocks per level */
    int64_t *block_start;       /* first block number of each level */
    int64_t *p8;
@@ -168,6 +170,8 @@ void amr_free(amr_run *R);
 int amr_init(amr_run *R);
 void amr_sorted_add(amr_run *R, int slot, int64_t number, int level);
 void amr_sorted_del(amr_run *R, int64_t number, int level);
+void amr_sorted_note(amr_run *R, int slot);
+void amr_sorted_rebuild(amr_run *R);
 int amr_n_active(const amr_run *R);
 double amr_rand01(amr_run *R);
 int amr_build_exchange_plan(amr_run *R);

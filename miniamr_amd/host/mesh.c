@@ -140,6 +140,9 @@ int amr_allocate(amr_run *R)
    R->scratch_i = (int *) amr_xmalloc(sizeof(int) * P->max_num_blocks);
    R->cellbuf = (double *) amr_xmalloc(sizeof(double) * (size_t) P->num_vars *
                                        (P->nx + 2) * (P->ny + 2) * (P->nz + 2));
+   R->sorted_new.elem = sizeof(int);
+   R->sorted_mark = (unsigned char *) amr_xmalloc(P->max_num_blocks);
+   memset(R->sorted_mark, 0, P->max_num_blocks);
    R->split_jobs.elem = sizeof(amr_split_job);
    R->merge_jobs.elem = sizeof(amr_merge_job);
    R->face_jobs.elem = sizeof(amr_face_job);
@@ -176,6 +179,7 @@ void amr_free(amr_run *R)
    free(R->sorted_index); free(R->num_blocks); free(R->block_start);
    free(R->p8); free(R->p2); free(R->grid_sum); free(R->scratch_i); free(R->cellbuf);
    free(R->split_jobs.v); free(R->merge_jobs.v); free(R->face_jobs.v);
+   free(R->sorted_new.v); free(R->sorted_mark);
    free(R->P.objects);
    memset(R, 0, sizeof(*R));
 }
@@ -200,6 +204,93 @@ void amr_sorted_add(amr_run *R, int slot, int64_t number, int level)
    R->sorted_slot[i] = slot;
    for (int l = level + 1; l <= last; l++)
       R->sorted_index[l]++;
+}
+
+/* Split and consolidate add and remove hundreds of blocks per pass.  They
+ * note each slot given a block (amr_sorted_note) and update the sorted list
+ * once at the end (amr_sorted_rebuild) instead of shifting its tail per
+ * change.  The list holds the active blocks level by level and, within a
+ * level, by number from largest to smallest; numbers are unique, so this
+ * is the list that amr_sorted_add / amr_sorted_del calls in any order over
+ * the same changes would leave (ref/block.c:610-646).  The rebuild keeps
+ * the old entries whose slot still holds the same block, sorts the noted
+ * ones (a few hundred) and merges the two, level by level. */
+typedef struct {
+   int64_t number;
+   int slot, level;
+} sorted_entry;
+
+static int cmp_sorted(const void *a, const void *b)
+{
+   const sorted_entry *x = (const sorted_entry *) a, *y = (const sorted_entry *) b;
+   if (x->level != y->level)
+      return x->level < y->level ? -1 : 1;
+   if (x->number != y->number)
+      return x->number > y->number ? -1 : 1;
+   return 0;
+}
+
+void amr_sorted_note(amr_run *R, int slot)
+{
+   amr_vec_push(&R->sorted_new, &slot);
+}
+
+void amr_sorted_rebuild(amr_run *R)
+{
+   const int last = R->P.num_refine + 1;
+   const int *noted = (const int *) R->sorted_new.v;
+   const int k = R->sorted_new.n;
+   /* the noted slots that still hold a block, each once, in list order */
+   sorted_entry *add = (sorted_entry *) amr_xmalloc(sizeof(sorted_entry) * (size_t) (k + 1));
+   int na = 0;
+   for (int t = 0; t < k; t++) {
+      const int s = noted[t];
+      if (R->sorted_mark[s])
+         continue;
+      R->sorted_mark[s] = 1;
+      if (R->blocks[s].number >= 0) {
+         add[na].number = R->blocks[s].number;
+         add[na].slot = s;
+         add[na].level = R->blocks[s].level;
+         na++;
+      }
+   }
+   qsort(add, (size_t) na, sizeof(sorted_entry), cmp_sorted);
+   const int old_n = R->sorted_index[last];
+   int64_t *num = (int64_t *) amr_xmalloc(sizeof(int64_t) * (size_t) (old_n + na + 1));
+   int *slot = (int *) amr_xmalloc(sizeof(int) * (size_t) (old_n + na + 1));
+   int n = 0, a = 0;
+   for (int l = 0; l < last; l++) {
+      const int i1 = R->sorted_index[l + 1];
+      int i = R->sorted_index[l];
+      R->sorted_index[l] = n;
+      for (;;) {
+         /* next surviving old entry of the level: its slot holds the same
+          * block and was not given another one in this pass */
+         while (i < i1 && (R->sorted_mark[R->sorted_slot[i]] ||
+                           R->blocks[R->sorted_slot[i]].number != R->sorted_number[i]))
+            i++;
+         const int has_new = a < na && add[a].level == l;
+         if (i < i1 && (!has_new || R->sorted_number[i] > add[a].number)) {
+            num[n] = R->sorted_number[i];
+            slot[n++] = R->sorted_slot[i++];
+         } else if (has_new) {
+            num[n] = add[a].number;
+            slot[n++] = add[a++].slot;
+         } else {
+            break;
+         }
+      }
+   }
+   R->sorted_index[last] = n;
+   memcpy(R->sorted_number, num, sizeof(int64_t) * (size_t) n);
+   memcpy(R->sorted_slot, slot, sizeof(int) * (size_t) n);
+   for (int t = 0; t < k; t++)
+      R->sorted_mark[noted[t]] = 0;
+   R->sorted_new.n = 0;
+   free(add);
+   free(num);
+   free(slot);
 }
 
 /* ref/block.c:628-646 */

@@ -229,6 +229,7 @@ void amr_split_blocks(amr_run *R)
 {
    const amr_params *P = &R->P;
    int cursor = 0;   /* first-fit cursor for child slots */
+   int pcursor = 0;  /* parents below it are in use (none is freed here) */
 
    for (int level = 0; level <= R->cur_max_level; level++) {
       /* the slot range grows while we go; new children are one level up */
@@ -247,15 +248,15 @@ void amr_split_blocks(amr_run *R)
                       P->max_num_blocks, 0);
          if (R->num_active + 8 > R->local_max_b)
             R->local_max_b = R->num_active + 8;
-         amr_sorted_del(R, b->number, level);
          R->num_active += 7;
          R->num_blocks[level]--;
          R->num_blocks[level + 1] += 8;
 
          int p;
-         for (p = 0; p < R->max_active_parent; p++)
+         for (p = pcursor; p < R->max_active_parent; p++)
             if (R->parents[p].number < 0)
                break;
+         pcursor = p + 1;
          if (p == R->max_num_parents)
             amr_fatal("ERROR: Need more parents\n");
          if (p == R->max_active_parent)
@@ -286,6 +287,7 @@ void amr_split_blocks(amr_run *R)
             if (m + 1 > R->max_active_block)
                R->max_active_block = m + 1;
             amr_block *c = &R->blocks[m];
+            amr_sorted_note(R, m);
             int ox = o % 2, oy = (o / 2) % 2, oz = o / 4;
             sib[o] = m;
             pp->child[o] = m;
@@ -295,7 +297,6 @@ void amr_split_blocks(amr_run *R)
             c->child_number = o;
             c->number = ((2 * zp + oz) * cy + (2 * yp + oy)) * cx + 2 * xp + ox +
                         R->block_start[level + 1];
-            amr_sorted_add(R, m, c->number, level + 1);
             c->cen[0] = b->cen[0] + (2 * ox - 1) * half;
             c->cen[1] = b->cen[1] + (2 * oy - 1) * half;
             c->cen[2] = b->cen[2] + (2 * oz - 1) * half;
@@ -353,6 +354,7 @@ void amr_split_blocks(amr_run *R)
       }
    }
    if (R->split_jobs.n) {
+      amr_sorted_rebuild(R);
       amr_dev_check(R, amr_dev_split(R->dev, (const amr_split_job *) R->split_jobs.v,
                                      R->split_jobs.n), "split");
       R->split_jobs.n = 0;
@@ -365,18 +367,21 @@ void amr_split_blocks(amr_run *R)
 void amr_consolidate_blocks(amr_run *R)
 {
    const amr_params *P = &R->P;
+   int cursor = 0;   /* every slot below it is in use: first fit from slot 0 */
    for (int level = R->cur_max_level; level >= 0; level--)
       for (int p = 0; p < R->max_active_parent; p++) {
          amr_parent *pp = &R->parents[p];
          if (pp->number < 0 || pp->level != level || pp->refine != -1)
             continue;
          int n;
-         for (n = 0; n < P->max_num_blocks; n++)
+         for (n = cursor; n < P->max_num_blocks; n++)
             if (R->blocks[n].number < 0)
                break;
          if (n == P->max_num_blocks)
             amr_fatal("Out of free blocks in consolidate_blocks %d\n", 0);
+         cursor = n + 1;
          amr_block *b = &R->blocks[n];
+         amr_sorted_note(R, n);
          if (n + 1 > R->max_active_block)
             R->max_active_block = n + 1;
          if (R->num_active + 1 > R->local_max_b)
@@ -393,13 +398,13 @@ void amr_consolidate_blocks(amr_run *R)
          b->child_number = pp->child_number;
          if (b->level)
             R->parents[b->parent].child[b->child_number] = n;
-         amr_sorted_add(R, n, b->number, level);
          b->refine = 0;
          memcpy(b->cen, pp->cen, sizeof(b->cen));
          for (int o = 0; o < 8; o++) {
             amr_block *c = &R->blocks[pp->child[o]];
-            amr_sorted_del(R, c->number, level + 1);
             c->number = -1;
+            if (pp->child[o] < cursor)
+               cursor = pp->child[o];
             amr_merge_job job = { n, pp->child[o], o, 0 };
             amr_vec_push(&R->merge_jobs, &job);
          }
@@ -431,6 +436,7 @@ void amr_consolidate_blocks(amr_run *R)
          }
       }
    if (R->merge_jobs.n) {
+      amr_sorted_rebuild(R);
       amr_dev_check(R, amr_dev_consolidate(R->dev, (const amr_merge_job *) R->merge_jobs.v,
                                            R->merge_jobs.n), "consolidate");
       R->merge_jobs.n = 0;
