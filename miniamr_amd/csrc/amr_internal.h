@@ -109,6 +109,13 @@ struct amr_dev {
    int elem_bytes;
    void *pool = nullptr;
    hipStream_t stream = nullptr;
+   // refinement marking (no cell data): its own stream, so that it runs
+   // beside the stages still queued on `stream` (default priority)
+   hipStream_t mark_stream = nullptr;
+   // pinned staging of split/consolidate job lists, copied in stream order;
+   // jobstage_used bytes may still feed queued copies until the stream drains
+   void *h_jobstage = nullptr;
+   size_t jobstage_bytes = 0, jobstage_used = 0;
    bool owns_stream = true;         // false for batch members (batch's stream)
    int plan_version = 0;            // bumped when device plan buffers change
 

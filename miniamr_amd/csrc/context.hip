@@ -177,6 +177,10 @@ int amr_dev_destroy(amr_dev *d)
       hipHostFree(d->h_plan);
    if (d->stream && d->owns_stream)
       hipStreamDestroy(d->stream);
+   if (d->mark_stream && d->mark_stream != d->stream)
+      hipStreamDestroy(d->mark_stream);
+   if (d->h_jobstage)
+      hipHostFree(d->h_jobstage);
    amr_tape_free(d);
    delete d;
    return AMR_OK;
@@ -869,13 +873,45 @@ static int refine_data(amr_dev *d, const J *jobs, int n, bool split)
    size_t per_job = (size_t) d->c.num_vars * (d->g.nx / 2) * (d->g.ny / 2) * (d->g.nz / 2) *
                     d->elem_bytes;
    int chunk = (int) std::max<size_t>(1, std::min<size_t>(n, kScratchCap / per_job));
-   int rc = ensure_scratch(d, per_job * chunk);
+   // A context of its own sizes both buffers once for the largest batch of
+   // jobs a refinement step can have (8 per slot): growing one waits for the
+   // device, stages still queued included.  Batch members, which share one
+   // stream and would each hold the scratch, size them as needed.
+   const size_t most = 8 * (size_t) d->c.max_blocks;
+   const size_t scratch_want =
+      d->owns_stream ? per_job * std::max<size_t>(chunk, std::min(most, kScratchCap / per_job))
+                     : per_job * chunk;
+   int rc = ensure_scratch(d, scratch_want);
    if (!rc)
-      rc = ensure_jobbuf(d, sizeof(J) * n);
+      rc = ensure_jobbuf(d, sizeof(J) * (d->owns_stream ? std::max<size_t>(n, most) : (size_t) n));
    if (rc)
       return rc;
-   HIP_TRY(d, hipStreamSynchronize(d->stream));
-   HIP_TRY(d, hipMemcpy(d->d_jobbuf, jobs, sizeof(J) * n, hipMemcpyHostToDevice));
+   // The job list goes through pinned staging with a copy ordered on the
+   // stream, so the host does not wait here for the stages still queued
+   // (the refinement's host work then overlaps them).  A stream that has
+   // drained frees the whole staging buffer; else the list is appended, and
+   // only when it does not fit does the host wait.
+   const size_t jb = (sizeof(J) * (size_t) n + 255) / 256 * 256;
+   if (d->jobstage_used && hipStreamQuery(d->stream) == hipSuccess)
+      d->jobstage_used = 0;
+   if (d->jobstage_used + jb > d->jobstage_bytes) {
+      HIP_TRY(d, hipStreamSynchronize(d->stream));
+      d->jobstage_used = 0;
+      if (jb > d->jobstage_bytes) {
+         if (d->h_jobstage)
+            hipHostFree(d->h_jobstage);
+         d->h_jobstage = nullptr;
+         d->jobstage_bytes = 0;
+         const size_t cap = std::max<size_t>(jb, 4 << 20);
+         HIP_TRY(d, hipHostMalloc(&d->h_jobstage, cap, hipHostMallocDefault));
+         d->jobstage_bytes = cap;
+      }
+   }
+   char *stage = (char *) d->h_jobstage + d->jobstage_used;
+   memcpy(stage, jobs, sizeof(J) * n);
+   d->jobstage_used += jb;
+   HIP_TRY(d, hipMemcpyAsync(d->d_jobbuf, stage, sizeof(J) * n, hipMemcpyHostToDevice,
+                             d->stream));
    amr_span_begin(d, 3);
    for (int t0 = 0; t0 < n && !rc; t0 += chunk) {
       int m = std::min(chunk, n - t0);

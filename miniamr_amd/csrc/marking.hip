@@ -221,10 +221,12 @@ __global__ __launch_bounds__(kMarkThreads) void k_mark_refine(MarkIn m)
 }
 
 template <typename X>
-int upload(amr_dev *d, void **buf, size_t *cap, const X *src, size_t n)
+int upload(amr_dev *d, void **buf, size_t *cap, const X *src, size_t n, size_t n_max)
 {
-   const size_t bytes = sizeof(X) * (n ? n : 1);
-   if (bytes > *cap) {
+   // sized once for the most the context can hold: freeing a device buffer
+   // waits for the whole device, stages still queued included
+   const size_t bytes = sizeof(X) * std::max<size_t>(std::max<size_t>(n, n_max), 1);
+   if (sizeof(X) * n > *cap) {
       hipFree(*buf);
       *buf = nullptr;
       *cap = 0;
@@ -232,7 +234,8 @@ int upload(amr_dev *d, void **buf, size_t *cap, const X *src, size_t n)
       *cap = bytes;
    }
    if (n)
-      HIP_TRY(d, hipMemcpyAsync(*buf, src, sizeof(X) * n, hipMemcpyHostToDevice, d->stream));
+      HIP_TRY(d, hipMemcpyAsync(*buf, src, sizeof(X) * n, hipMemcpyHostToDevice,
+                                d->mark_stream));
    return AMR_OK;
 }
 
@@ -277,11 +280,21 @@ extern "C" int amr_dev_mark_refine(amr_dev *d, const amr_mark_params *p,
             return amr_set_error(d, AMR_ERR_ARG, "mark_refine: child of parent %d", q);
 
    static_assert(sizeof(amr_mark_block) % 4 == 0, "block record");
+   // Marking reads no cell data: it runs on a stream of its own, beside the
+   // stages that may still be queued on d->stream (batch members, which
+   // share the batch's stream, mark on that stream).
+   if (!d->mark_stream) {
+      if (d->owns_stream)
+         HIP_TRY(d, hipStreamCreateWithFlags(&d->mark_stream, hipStreamNonBlocking));
+      else
+         d->mark_stream = d->stream;
+   }
    int rc;
-   if ((rc = upload(d, &d->mk_blocks, &d->mk_blocks_cap, blocks, n_blocks)) ||
-       (rc = upload(d, &d->mk_parents, &d->mk_parents_cap, parents, n_parents)) ||
-       (rc = upload(d, &d->mk_objects, &d->mk_objects_cap, objects, p->n_objects)) ||
-       (rc = upload(d, &d->mk_levels, &d->mk_levels_cap, level_start, p->num_refine + 2)))
+   const size_t mb = (size_t) d->c.max_blocks;
+   if ((rc = upload(d, &d->mk_blocks, &d->mk_blocks_cap, blocks, n_blocks, mb)) ||
+       (rc = upload(d, &d->mk_parents, &d->mk_parents_cap, parents, n_parents, mb)) ||
+       (rc = upload(d, &d->mk_objects, &d->mk_objects_cap, objects, p->n_objects, 64)) ||
+       (rc = upload(d, &d->mk_levels, &d->mk_levels_cap, level_start, p->num_refine + 2, 64)))
       return rc;
    const size_t flag_bytes = (size_t) n_blocks + n_parents;
    int8_t *out = nullptr;
@@ -293,9 +306,10 @@ extern "C" int amr_dev_mark_refine(amr_dev *d, const amr_mark_params *p,
          o = nullptr;
          d->mk_out = nullptr;
          d->mk_out_cap = 0;
-         HIP_TRY(d, hipMalloc(&o, flag_bytes));
+         const size_t want = std::max(flag_bytes, 2 * mb);
+         HIP_TRY(d, hipMalloc(&o, want));
          d->mk_out = o;
-         d->mk_out_cap = flag_bytes;
+         d->mk_out_cap = want;
       }
       out = (int8_t *) d->mk_out;
    }
@@ -312,12 +326,14 @@ extern "C" int amr_dev_mark_refine(amr_dev *d, const amr_mark_params *p,
    m.nz = d->g.nz;
    m.out_blocks = out;
    m.out_parents = out + n_blocks;
-   hipLaunchKernelGGL(k_mark_refine, dim3(1), dim3(kMarkThreads), flag_bytes, d->stream, m);
+   hipLaunchKernelGGL(k_mark_refine, dim3(1), dim3(kMarkThreads), flag_bytes, d->mark_stream, m);
    HIP_TRY(d, hipGetLastError());
-   HIP_TRY(d, hipStreamSynchronize(d->stream));
    if (n_blocks)
-      HIP_TRY(d, hipMemcpy(block_refine, out, n_blocks, hipMemcpyDeviceToHost));
+      HIP_TRY(d, hipMemcpyAsync(block_refine, out, n_blocks, hipMemcpyDeviceToHost,
+                                d->mark_stream));
    if (n_parents)
-      HIP_TRY(d, hipMemcpy(parent_refine, out + n_blocks, n_parents, hipMemcpyDeviceToHost));
+      HIP_TRY(d, hipMemcpyAsync(parent_refine, out + n_blocks, n_parents, hipMemcpyDeviceToHost,
+                                d->mark_stream));
+   HIP_TRY(d, hipStreamSynchronize(d->mark_stream));
    return AMR_OK;
 }

@@ -85,6 +85,7 @@ typedef struct { void *v; int n, cap, elem; } amr_vec;
 typedef struct {
    int ts, comm_stage, state, n_rows;
    int stage_pos;              /* stages of timestep ts done by amr_driver_stages */
+   int checked;                /* timestep ts's checksums already checked: 1 passed, 2 failed */
    double delta, sim_time, t_start;
    double dev_ms[4];
    int64_t stage_calls;

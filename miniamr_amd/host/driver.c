@@ -332,26 +332,37 @@ void amr_driver_step_advance(amr_run *R)
    R->loop.comm_stage += R->P.stages_per_ts;
 }
 
-int amr_driver_step_end(amr_run *R)
+/* Read and check timestep ts's checksums (ref/driver.c:88-103).  Returns 1
+ * if the run must stop. */
+static int read_and_check(amr_run *R)
 {
-   amr_params *P = &R->P;
    amr_loop *L = &R->loop;
-   int ts = L->ts;
    if (L->n_rows)
       amr_dev_check(R, amr_dev_read_sums(R->dev, 0, L->n_rows, L->sums), "read_sums");
    else
       amr_dev_check(R, amr_dev_sync(R->dev), "sync");
    double t3 = amr_now();
-   double ms[4];
-   amr_dev_read_timing(R->dev, ms);
-   for (int i = 0; i < 4; i++)
-      L->dev_ms[i] += ms[i];
-   if (check_timestep(R, ts, L->sums, &R->total_blocks)) {
+   int stop = check_timestep(R, L->ts, L->sums, &R->total_blocks);
+   R->t_cs_host += amr_now() - t3;
+   return stop;
+}
+
+int amr_driver_step_end(amr_run *R)
+{
+   amr_params *P = &R->P;
+   amr_loop *L = &R->loop;
+   int ts = L->ts;
+   int stop = L->checked ? L->checked == 2 : read_and_check(R);
+   L->checked = 0;
+   if (stop) {
+      double ms[4];
+      amr_dev_read_timing(R->dev, ms);
+      for (int i = 0; i < 4; i++)
+         L->dev_ms[i] += ms[i];
       L->state = 2;
       return 2;
    }
    double t4 = amr_now();
-   R->t_cs_host += t4 - t3;
 
    if (P->num_refine && !P->uniform_refine) {
       amr_move_objects(R, L->delta);
@@ -365,6 +376,12 @@ int amr_driver_step_end(amr_run *R)
    }
    amr_dev_check(R, amr_dev_sync(R->dev), "sync");
    R->t_refine_all += amr_now() - t4;
+   /* device time of the timestep's spans (and of the refinement's data
+    * work, which the next timestep would otherwise count) */
+   double ms[4];
+   amr_dev_read_timing(R->dev, ms);
+   for (int i = 0; i < 4; i++)
+      L->dev_ms[i] += ms[i];
 
    L->sim_time += L->delta;
    int done = 0;
@@ -383,12 +400,41 @@ int amr_driver_step_end(amr_run *R)
    return L->state;
 }
 
+/* The stage after which a timestep that ends in a refinement checks its
+ * checksums, or -1.  Refinement depends on the objects and the mesh only,
+ * never on cell values, so once the last checksum of the timestep has been
+ * checked (ref/driver.c:88-103: the test that may stop the run comes
+ * first) its host work -- marking, the split and consolidate loops, the new
+ * mesh's plans -- can run while the device still works through the
+ * timestep's remaining stages, instead of leaving the device idle after
+ * them.  Only the stage-major order, and only when stages follow the last
+ * checksum stage. */
+static int early_check_stage(const amr_run *R)
+{
+   const amr_params *P = &R->P;
+   if (P->schedule != 0 || !R->loop.n_rows || !P->num_refine || P->uniform_refine ||
+       (R->loop.ts % P->refine_freq))
+      return -1;
+   const int last = (P->stages_per_ts - 1) / P->checksum_freq * P->checksum_freq;
+   return last < P->stages_per_ts - 1 ? last : -1;
+}
+
 int amr_driver_step(amr_run *R)
 {
    int st = amr_driver_step_begin(R);
    if (st)
       return st;
-   amr_driver_step_enqueue(R);
+   const int c = early_check_stage(R);
+   if (c >= 0) {
+      amr_loop *L = &R->loop;
+      enqueue_stage_range(R, L->comm_stage, 0, c + 1);
+      L->checked = read_and_check(R) ? 2 : 1;
+      /* the remaining stages run whatever the check said, as when the
+       * whole timestep is checked at its end */
+      enqueue_stage_range(R, L->comm_stage, c + 1, R->P.stages_per_ts);
+   } else {
+      amr_driver_step_enqueue(R);
+   }
    amr_driver_step_advance(R);
    return amr_driver_step_end(R);
 }
