@@ -48,8 +48,11 @@ typedef struct {
                             1: variable-major within a refinement interval */
    int device;
    int device_marking;   /* MINIAMR_DEVICE_MARKING: -1 (unset) refinement
-                            marking on the device when it accepts the mesh,
-                            else on the host; 0 host; 1 device or fail */
+                            marking on the host when the refinement overlaps
+                            the timestep's last stages (no device round
+                            trips beside the queued stages), else on the
+                            device when it accepts the mesh, else on the
+                            host; 0 host; 1 device or fail */
    int seed;             /* --seed: generator seed of the initial values
                             (1 = the reference's implicit srand(1)) */
    int tape;             /* --tape: record data operations from allocation on */
@@ -86,6 +89,7 @@ typedef struct {
    int ts, comm_stage, state, n_rows;
    int stage_pos;              /* stages of timestep ts done by amr_driver_stages */
    int checked;                /* timestep ts's checksums already checked: 1 passed, 2 failed */
+   int overlapped;             /* the refinement in progress overlaps queued stages */
    double delta, sim_time, t_start;
    double dev_ms[4];
    int64_t stage_calls;

@@ -353,12 +353,14 @@ int amr_driver_step_end(amr_run *R)
    amr_loop *L = &R->loop;
    int ts = L->ts;
    int stop = L->checked ? L->checked == 2 : read_and_check(R);
+   L->overlapped = L->checked != 0;
    L->checked = 0;
    if (stop) {
       double ms[4];
       amr_dev_read_timing(R->dev, ms);
       for (int i = 0; i < 4; i++)
          L->dev_ms[i] += ms[i];
+      L->overlapped = 0;
       L->state = 2;
       return 2;
    }
@@ -374,6 +376,7 @@ int amr_driver_step_end(amr_run *R)
             R->nb_max = R->global_active;
       }
    }
+   L->overlapped = 0;
    amr_dev_check(R, amr_dev_sync(R->dev), "sync");
    R->t_refine_all += amr_now() - t4;
    /* device time of the timestep's spans (and of the refinement's data

@@ -160,7 +160,7 @@ int amr_refine_level(amr_run *R)
 static int device_mark(amr_run *R)
 {
    const amr_params *P = &R->P;
-   if (P->device_marking == 0)
+   if (P->device_marking == 0 || (P->device_marking < 0 && R->loop.overlapped))
       return 0;
    const int na = amr_n_active(R), np = R->max_active_parent;
    amr_mark_block *mb = (amr_mark_block *) amr_xmalloc(sizeof(amr_mark_block) * (na ? na : 1));
