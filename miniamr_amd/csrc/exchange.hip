@@ -54,10 +54,14 @@ __global__ __launch_bounds__(NT) void k_exchange_fused(T *__restrict__ pool,
 
 // Variables and threads per workgroup.  Default 8 x 256 (measured best for
 // FP64 and FP32 on the 16^3 workload: 8 x 128 was 3% slower in FP32, 16
-// variables 1-3% slower; in fused stages 16 variables were 6% slower in
-// FP64 and 2% faster in FP32, 4 variables 11% / 35% slower);
-// MINIAMR_EXCHANGE ("vars,threads" with vars 4/8/16 and threads 128/256)
-// overrides it (fused stages take the variable count, at 256 threads).
+// variables 1-3% slower).  Fused stages, whose jobs are mostly coarse/fine
+// faces: 8 variables in FP64 (16 were 6% slower, 4 11% slower), 20 in
+// FP32 (exchange 5% shorter than with 16; 8 and 4 slower) and 40 in BF16
+// (exchange 17% shorter than with 8; 16 and 20 in between): the narrower
+// the type, the more variables a workgroup's loads need to fill the
+// memory pipeline.  MINIAMR_EXCHANGE ("vars,threads" with vars
+// 4/8/16/20/40 and threads 128/256) overrides it (fused stages take the
+// variable count, at 256 threads; plain stages 16 for 20 and 40).
 struct ExShape {
    int kv = kVars, nt = 256;
    bool chosen = false;   // set by MINIAMR_EXCHANGE
@@ -71,7 +75,7 @@ ExShape ex_shape()
       read = true;
       const char *e = getenv("MINIAMR_EXCHANGE");
       int kv = 0, nt = 0;
-      if (e && sscanf(e, "%d,%d", &kv, &nt) == 2 && (kv == 4 || kv == 8 || kv == 16) &&
+      if (e && sscanf(e, "%d,%d", &kv, &nt) == 2 && (kv == 4 || kv == 8 || kv == 16 || kv == 20 || kv == 40) &&
           (nt == 128 || nt == 256)) {
          sh.kv = kv;
          sh.nt = nt;
@@ -103,10 +107,10 @@ int launch(amr_dev *d, int dir, int start, int num)
       return AMR_OK;
    const ExShape sh = ex_shape();
    if (sh.nt == 256)
-      return sh.kv == 16 ? launch_kv<T, 16, 256>(d, off, n, start, num)
+      return sh.kv >= 16 ? launch_kv<T, 16, 256>(d, off, n, start, num)
            : sh.kv == 4 ? launch_kv<T, 4, 256>(d, off, n, start, num)
                         : launch_kv<T, 8, 256>(d, off, n, start, num);
-   return sh.kv == 16 ? launch_kv<T, 16, 128>(d, off, n, start, num)
+   return sh.kv >= 16 ? launch_kv<T, 16, 128>(d, off, n, start, num)
         : sh.kv == 4 ? launch_kv<T, 4, 128>(d, off, n, start, num)
                      : launch_kv<T, 8, 128>(d, off, n, start, num);
 }
@@ -121,11 +125,21 @@ int launch_fused(amr_dev *d, int dir, int start, int num)
    Lay L;
    if (int rc = amr_lay(d, start, num, &L))
       return rc;
-   // 16 variables per item for FP32 in fused stages (2% faster there; FP64
-   // 6% slower), unless MINIAMR_EXCHANGE chooses
+   // variables per item by storage type (see ExShape), unless
+   // MINIAMR_EXCHANGE chooses
    const ExShape sh = ex_shape();
-   const int kv = sh.chosen ? sh.kv : sizeof(T) == 4 ? 16 : kVars;
-   if (kv == 16) {
+   const int kv = sh.chosen ? sh.kv : sizeof(T) == 4 ? 20 : sizeof(T) == 2 ? 40 : kVars;
+   if (kv == 20) {
+      dim3 grid(n, (num + 19) / 20);
+      hipLaunchKernelGGL((k_exchange_fused<T, 20, 256>), grid, dim3(256), 0, d->stream,
+                         (T *) d->pool, d->d_fjobs + off, d->d_edges, L, start,
+                         start + num);
+   } else if (kv == 40) {
+      dim3 grid(n, (num + 39) / 40);
+      hipLaunchKernelGGL((k_exchange_fused<T, 40, 256>), grid, dim3(256), 0, d->stream,
+                         (T *) d->pool, d->d_fjobs + off, d->d_edges, L, start,
+                         start + num);
+   } else if (kv == 16) {
       dim3 grid(n, (num + 15) / 16);
       hipLaunchKernelGGL((k_exchange_fused<T, 16, 256>), grid, dim3(256), 0, d->stream,
                          (T *) d->pool, d->d_fjobs + off, d->d_edges, L, start,
