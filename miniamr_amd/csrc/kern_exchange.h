@@ -164,6 +164,56 @@ __device__ __forceinline__ void exchange_job(T *__restrict__ pool, const amr_fac
                }
          }
       }
+   } else if (!whole && !(nb & 1)) {
+      // RESTRICT of the face interior with paired loads: a coarse cell's
+      // terms in the reference's order are (fa, fb), (fa, fb+1), (fa+1, fb),
+      // (fa+1, fb+1); each row's two are adjacent in the source plane (b is
+      // its fastest axis, and fb - 1 = 2(b - bo) - 2 is even, as is every
+      // part and record start), so one two-element load gives terms 0 and
+      // 1 and another terms 2 and 3, summed in the same order.
+      typedef typename VecOf2<T>::type V2;
+      constexpr int RV = CP * KV / 4 < 1 ? 1 : CP * KV / 4;
+      const int ha = na / 2, hb = nb / 2, ncell = ha * hb;
+      const int ao = jb.jq * ha, bo = jb.iq * hb;
+      const int G = var_groups<KV>(ncell, nthr), cw = nthr / G;
+      const int g = __builtin_amdgcn_readfirstlane(tid / cw), c0 = tid - g * cw;
+      for (int t = c0; t < ncell; t += cw) {
+         const int a = 1 + ao + t / hb, b = 1 + bo + t % hb;   // coarse ghost cell
+         const int fa = 2 * (a - ao) - 1, fb = 2 * (b - bo) - 1;
+         int si, sj, sk;
+         canon(dir, plane, fa, fb, si, sj, sk);
+         const int s0 = dir == 2 ? zcopy(L, si, sj, sk) : home(L, si, sj, sk);
+         canon(dir, plane, fa + 1, fb, si, sj, sk);
+         const int s1 = dir == 2 ? zcopy(L, si, sj, sk) : home(L, si, sj, sk);
+         int i, j, k;
+         canon(dir, ghost, a, b, i, j, k);
+         const int dh = home(L, i, j, k);
+         const int dc = (k == 1 || k == L.nz) ? zcopy(L, i, j, k) : -1;
+         for (int q0 = g; q0 < nv; q0 += RV * G) {
+            V2 v[RV][2];
+#pragma unroll
+            for (int m = 0; m < RV; m++)
+               if (q0 + m * G < nv) {
+                  const gptr<const T> sq = uniform_base(src + (int64_t) (q0 + m * G) * vs);
+                  v[m][0] = *(gptr<const V2>) ((gptr<const char>) sq +
+                                               (uint32_t) s0 * (uint32_t) sizeof(T));
+                  v[m][1] = *(gptr<const V2>) ((gptr<const char>) sq +
+                                               (uint32_t) s1 * (uint32_t) sizeof(T));
+               }
+#pragma unroll
+            for (int m = 0; m < RV; m++)
+               if (q0 + m * G < nv) {
+                  const gptr<T> dq = uniform_base(dst + (int64_t) (q0 + m * G) * vs);
+                  A s = to_acc((T) v[m][0][0]);
+                  s = s + to_acc((T) v[m][0][1]);
+                  s = s + to_acc((T) v[m][1][0]);
+                  s = s + to_acc((T) v[m][1][1]);
+                  const T o = from_acc<T>(s);
+                  st(dq, dh, o);
+                  if (dc >= 0) st(dq, dc, o);
+               }
+         }
+      }
    } else {  // RESTRICT: coarse quarter cell <- sum of 4 fine cells
       // RV of the thread's variables per pass (4 loads each)
       constexpr int RV = CP * KV / 4 < 1 ? 1 : CP * KV / 4;
