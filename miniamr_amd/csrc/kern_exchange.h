@@ -314,14 +314,53 @@ __device__ __forceinline__ void exchange_job_fused(T *__restrict__ pool, const F
    }
    const int a0 = 1 - ext_a, b0 = 1 - ext_b;
    const int wa = na + 2 * ext_a, wb = nb + 2 * ext_b;
-   const bool ext_only = fj.mode == FJ_EXT_ONLY;
-   // ext-only: rows ai = 0 and wa-1 (all bi), then, with ext_b, columns
-   // bi = 0 and wb-1 of the rows between
-   const int ncell = ext_only ? 2 * wb + (ext_b ? 2 * (wa - 2) : 0) : wa * wb;
    const int sp = jb.kind == AMR_FACE_BC ? (hi ? n : 1) : (hi ? 1 : n);
    const int64_t vs = L.var_stride;
    const gptr<T> base = global_view(pool) + (int64_t) v0 * vs;
    const int64_t dst0 = (int64_t) jb.dst * L.rec;
+   bool ext_only = fj.mode == FJ_EXT_ONLY;
+   if (!ext_only && !(nb & 1) && (ext_a || !ext_b)) {
+      // A full job's face-interior cells in pairs along b.  Their sources
+      // are cells of the source block's own plane (no ring row, so no
+      // alt[] block and no edge entry), adjacent like the ghost cells they
+      // fill, and pairs start on even elements: one two-element load and
+      // store per variable.  The ring cells then go one by one, as in an
+      // ext-only job.
+      typedef typename VecOf2<T>::type V2;
+      const int hb = nb / 2, npair = na * hb;
+      const int Gp = var_groups<KV>(npair, nthr), cwp = nthr / Gp;
+      const int gp = __builtin_amdgcn_readfirstlane(tid / cwp), cp = tid - gp * cwp;
+      const int64_t src0 = (int64_t) jb.src * L.rec;
+      for (int t = cp; t < npair; t += cwp) {
+         const int a = 1 + t / hb, m = t - (a - 1) * hb, b = 2 * m + 1;
+         int i, j, k, i1, j1, k1, si, sj, sk;
+         canon(dir, sp, a, b, si, sj, sk);
+         canon(dir, ghost, a, b, i, j, k);
+         canon(dir, ghost, a, b + 1, i1, j1, k1);
+         const int64_t src = src0 + (dir == 2 ? zcopy(L, si, sj, sk) : home(L, si, sj, sk));
+         const int64_t dh = dst0 + home(L, i, j, k);   // (a, b + 1) is dh + 1
+         const int64_t dc0 = (k == 1 || k == L.nz) ? dst0 + zcopy(L, i, j, k) : -1;
+         const int64_t dc1 = (k1 == 1 || k1 == L.nz) ? dst0 + zcopy(L, i1, j1, k1) : -1;
+         V2 v[KV];
+#pragma unroll
+         for (int q = 0; q < KV; q++)
+            if (q < nv && (q & (Gp - 1)) == gp)
+               v[q] = *(gptr<const V2>) (base + (q * vs + src));
+#pragma unroll
+         for (int q = 0; q < KV; q++)
+            if (q < nv && (q & (Gp - 1)) == gp) {
+               *(gptr<V2>) (base + (q * vs + dh)) = v[q];
+               if (dc0 >= 0) base[q * vs + dc0] = v[q][0];
+               if (dc1 >= 0) base[q * vs + dc1] = v[q][1];
+            }
+      }
+      if (!ext_a)
+         return;   // no ring: an x face, or the 7-point stencil
+      ext_only = true;
+   }
+   // ext-only: rows ai = 0 and wa-1 (all bi), then, with ext_b, columns
+   // bi = 0 and wb-1 of the rows between
+   const int ncell = ext_only ? 2 * wb + (ext_b ? 2 * (wa - 2) : 0) : wa * wb;
    const int G = var_groups<KV>(ncell, nthr), cw = nthr / G;
    const int g = __builtin_amdgcn_readfirstlane(tid / cw), c0 = tid - g * cw;
    for (int t = c0; t < ncell; t += cw) {
